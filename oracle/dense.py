@@ -1,0 +1,527 @@
+"""
+Dense CPU restatement of the reference D4 energy path (TEST INFRASTRUCTURE
+ONLY; see ``oracle/__init__.py``).
+
+Every function mirrors a reference function op for op; each docstring cites
+the reference file:line it follows (paths relative to the reference
+checkout). Stages that the reference takes from absent dependencies are
+restated from their published algorithms:
+
+- ``cn_d4`` / ``erf_count`` / ``cn_eeq`` / ``cut_cn``: tad-mctc 0.7.0
+  (call sites ``dispersion/base.py:390``, ``disp.py:30``; constants mirrored
+  at ``defaults.py:26-50``).
+- ``eeq_charges``: tad-multicharge 0.5.0 ``get_eeq_charges`` (call site
+  ``dispersion/base.py:401-407``).
+
+Parity unpinned (no golden vector, closed form or invariant in the reference
+fixes them): the element tables of ``tad_dftd4_amd.data.elements`` for
+elements other than H, Li, C, N, O, Si, S; the tad-mctc default CN cutoff
+(30 bohr here; numerically inert beyond ~20 bohr for Z <= 54); the
+``storch.sqrt`` floor for an EEQ coordination number of exactly zero
+(isolated atoms only; ``sqrt(max(x, eps))`` here).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from tad_dftd4_amd.data import elements as _el
+from tad_dftd4_amd.data import ref_tables as _ref_tables
+
+Tensor = torch.Tensor
+
+# defaults.py:26-93
+D4_CN_CUTOFF = 30.0
+D4_CN_EEQ_CUTOFF = 25.0
+D4_CN_EEQ_MAX = 8.0
+D4_DISP2_CUTOFF = 60.0
+D4_DISP3_CUTOFF = 40.0
+D4_KCN = 7.5
+D4_K4 = 4.10451
+D4_K5 = 19.08857
+D4_K6 = 2 * 11.28174**2
+S6, S8, S9, RS9, ALP, A1, A2 = 1.0, 1.0, 1.0, 1.0, 16.0, 0.4, 5.0
+# model/base.py:57-59
+GA, GC, WF = 3.0, 2.0, 6.0
+
+
+def _t(x, dtype=torch.float64) -> Tensor:
+    return torch.as_tensor(np.asarray(x), dtype=dtype)
+
+
+class Tables:
+    """Element tables as torch tensors (built once)."""
+
+    def __init__(self, dtype=torch.float64):
+        r = _ref_tables()
+        self.refcovcn = _t(r["refcovcn"])
+        self.refalpha = _t(r["refalpha"], dtype)
+        self.refascale = _t(r["refascale"], dtype)
+        self.refscount = _t(r["refscount"], dtype)
+        self.refsys = _t(r["refsys"], torch.long)
+        self.refc = _t(r["refc"], torch.long)
+        self.secscale = _t(r["secscale"], dtype)
+        self.secalpha = _t(r["secalpha"], dtype)
+        self.clsq = _t(r["clsq"], dtype)
+        self.clsh = _t(r["clsh"], dtype)
+        self.cp_weights = _t(r["cp_weights"], dtype)
+        raw = _t(r["r4_over_r2"], dtype)
+        # data/r4r2.py:84-88
+        sqrtz = torch.sqrt(torch.arange(len(raw), dtype=dtype))
+        self.r4r2 = torch.sqrt(0.5 * (raw * sqrtz))
+        self.cov_d3 = _t(_el.COV_D3, dtype)
+        self.en = _t(_el.PAULING_EN, dtype)
+        self.gam = _t(_el.GAM, dtype)
+        self.zeff = _t(_el.ZEFF, dtype)
+        self.chi = _t(_el.EEQ_CHI, dtype)
+        self.eta = _t(_el.EEQ_ETA, dtype)
+        self.kcn = _t(_el.EEQ_KCN, dtype)
+        self.rad = _t(_el.EEQ_RAD, dtype)
+
+
+_TABLES: dict[torch.dtype, Tables] = {}
+
+
+def tables(dtype=torch.float64) -> Tables:
+    if dtype not in _TABLES:
+        _TABLES[dtype] = Tables(dtype)
+    return _TABLES[dtype]
+
+
+# --------------------------------------------------------------------------
+# tad-mctc helpers (masks, safe math, distances)
+# --------------------------------------------------------------------------
+
+
+def real_atoms(numbers: Tensor) -> Tensor:
+    return numbers != 0
+
+
+def real_pairs(numbers: Tensor, mask_diagonal: bool = True) -> Tensor:
+    """tad-mctc ``batch.real_pairs`` (used at ``twobody.py:140``)."""
+    real = real_atoms(numbers)
+    mask = real.unsqueeze(-2) * real.unsqueeze(-1)
+    if mask_diagonal:
+        eye = torch.eye(numbers.shape[-1], dtype=torch.bool)
+        mask = mask * ~eye
+    return mask
+
+
+def real_triples(numbers: Tensor) -> Tensor:
+    """tad-mctc ``batch.real_triples(mask_diagonal=True, mask_self=True)``
+    (used at ``threebody.py:99``): all three atoms real and distinct."""
+    real = real_pairs(numbers, mask_diagonal=False)
+    mask = real.unsqueeze(-3) * real.unsqueeze(-2) * real.unsqueeze(-1)
+    n = numbers.shape[-1]
+    eye = torch.eye(n, dtype=torch.bool)
+    mask = mask * ~eye.unsqueeze(-1) * ~eye.unsqueeze(-2) * ~eye.unsqueeze(-3)
+    return mask
+
+
+def cdist(positions: Tensor) -> Tensor:
+    """Pairwise distances (tad-mctc ``storch.cdist``, p=2). The diagonal is
+    clamped to eps so that autograd stays finite; callers mask it anyway."""
+    diff = positions.unsqueeze(-2) - positions.unsqueeze(-3)
+    eps = torch.finfo(positions.dtype).eps
+    return torch.sqrt(torch.clamp(torch.sum(diff * diff, dim=-1), min=eps))
+
+
+def safe_sqrt(x: Tensor) -> Tensor:
+    """tad-mctc ``storch.sqrt``: square root with the argument floored at
+    eps (gradient-safe at zero)."""
+    return torch.sqrt(torch.clamp(x, min=torch.finfo(x.dtype).eps))
+
+
+def safe_divide(x: Tensor, y: Tensor) -> Tensor:
+    """tad-mctc ``storch.divide``: replaces exact zeros of the denominator
+    by eps."""
+    eps = torch.finfo(y.dtype).eps
+    return x / torch.where(y == 0, torch.full_like(y, eps), y)
+
+
+# --------------------------------------------------------------------------
+# coordination numbers (tad-mctc)
+# --------------------------------------------------------------------------
+
+
+def erf_count(r: Tensor, r0: Tensor, kcn: float = D4_KCN) -> Tensor:
+    """tad-mctc ``ncoord.count.erf_count``: 1/2 (1 + erf(-k (r/r0 - 1)))."""
+    return 0.5 * (1.0 + torch.erf(-kcn * (r / r0 - 1.0)))
+
+
+def cn_d4(numbers: Tensor, positions: Tensor, cutoff: float = D4_CN_CUTOFF):
+    """tad-mctc ``ncoord.cn_d4`` with ``erf_count`` (call site
+    ``dispersion/base.py:390``; constants ``defaults.py:41-50``)."""
+    t = tables(positions.dtype)
+    dtype = positions.dtype
+    mask = real_pairs(numbers, mask_diagonal=True)
+    eps = torch.tensor(torch.finfo(dtype).eps, dtype=dtype)
+    distances = torch.where(mask, cdist(positions), eps)
+    rcov = t.cov_d3[numbers]
+    en = t.en[numbers]
+    rc = rcov.unsqueeze(-2) + rcov.unsqueeze(-1)
+    endiff = torch.abs(en.unsqueeze(-2) - en.unsqueeze(-1))
+    den = D4_K4 * torch.exp(-((endiff + D4_K5) ** 2) / D4_K6)
+    cf = torch.where(
+        mask * (distances <= cutoff),
+        den * erf_count(distances, rc, D4_KCN),
+        torch.tensor(0.0, dtype=dtype),
+    )
+    return torch.sum(cf, dim=-1)
+
+
+def cut_cn(cn: Tensor, cn_max: float = D4_CN_EEQ_MAX) -> Tensor:
+    """tad-mctc ``cut_coordination_number``: smooth cap of the EEQ CN."""
+    cmax = torch.tensor(cn_max, dtype=cn.dtype)
+    return torch.log(1.0 + torch.exp(cmax)) - torch.log(1.0 + torch.exp(cmax - cn))
+
+
+def cn_eeq(numbers: Tensor, positions: Tensor, cutoff: float = D4_CN_EEQ_CUTOFF,
+           cn_max: float = D4_CN_EEQ_MAX) -> Tensor:
+    """tad-mctc ``ncoord.cn_eeq`` (erf counting, kcn = 7.5, capped)."""
+    t = tables(positions.dtype)
+    dtype = positions.dtype
+    mask = real_pairs(numbers, mask_diagonal=True)
+    eps = torch.tensor(torch.finfo(dtype).eps, dtype=dtype)
+    distances = torch.where(mask, cdist(positions), eps)
+    rcov = t.cov_d3[numbers]
+    rc = rcov.unsqueeze(-2) + rcov.unsqueeze(-1)
+    cf = torch.where(
+        mask * (distances <= cutoff),
+        erf_count(distances, rc, D4_KCN),
+        torch.tensor(0.0, dtype=dtype),
+    )
+    return cut_cn(torch.sum(cf, dim=-1), cn_max)
+
+
+# --------------------------------------------------------------------------
+# EEQ charges (tad-multicharge)
+# --------------------------------------------------------------------------
+
+
+def eeq_matrix(numbers: Tensor, positions: Tensor, total_charge: Tensor,
+               cn: Tensor) -> tuple[Tensor, Tensor]:
+    """Bordered EEQ system of tad-multicharge ``EEQModel.solve``: Coulomb
+    block erf(gamma r)/r with gamma = (rho_i^2 + rho_j^2)^-1/2, diagonal
+    eta + sqrt(2/pi)/rho, a border of ones for real atoms, and the right-hand
+    side -chi + kappa sqrt(cn) | Q. Padding rows get a unit diagonal."""
+    t = tables(positions.dtype)
+    dtype = positions.dtype
+    zero = torch.tensor(0.0, dtype=dtype)
+    eps = torch.tensor(torch.finfo(dtype).eps, dtype=dtype)
+    stop = math.sqrt(2.0 / math.pi)
+
+    real = real_atoms(numbers)
+    mask = real_pairs(numbers, mask_diagonal=True)
+    distances = torch.where(mask, cdist(positions), eps)
+    diagonal = torch.eye(numbers.shape[-1], dtype=torch.bool).expand_as(mask)
+
+    cc = torch.where(real, -t.chi[numbers] + safe_sqrt(cn) * t.kcn[numbers], zero)
+    rhs = torch.cat((cc, total_charge.unsqueeze(-1)), dim=-1)
+
+    rad = t.rad[numbers]
+    rads = rad.unsqueeze(-1) ** 2 + rad.unsqueeze(-2) ** 2
+    gamma = torch.where(mask, 1.0 / torch.sqrt(torch.where(mask, rads, 1.0)), zero)
+    eta = torch.where(
+        real, t.eta[numbers] + stop / torch.where(real, rad, 1.0),
+        torch.tensor(1.0, dtype=dtype),
+    )
+    coulomb = torch.where(
+        diagonal,
+        eta.unsqueeze(-1).expand_as(mask),
+        torch.where(mask, torch.erf(distances * gamma) / distances, zero),
+    )
+    constraint = real.to(dtype)
+    zeros = torch.zeros(numbers.shape[:-1], dtype=dtype)
+    matrix = torch.cat(
+        (
+            torch.cat((coulomb, constraint.unsqueeze(-1)), dim=-1),
+            torch.cat((constraint, zeros.unsqueeze(-1)), dim=-1).unsqueeze(-2),
+        ),
+        dim=-2,
+    )
+    return matrix, rhs
+
+
+def eeq_charges(numbers: Tensor, positions: Tensor, total_charge: Tensor,
+                cutoff: float = D4_CN_EEQ_CUTOFF) -> Tensor:
+    """tad-multicharge ``get_eeq_charges(numbers, positions, charge,
+    cutoff=cutoff.cn_eeq)`` (``dispersion/base.py:401-407``)."""
+    cn = cn_eeq(numbers, positions, cutoff=cutoff)
+    matrix, rhs = eeq_matrix(numbers, positions, total_charge, cn)
+    x = torch.linalg.solve(matrix, rhs.unsqueeze(-1)).squeeze(-1)
+    return x[..., :-1]
+
+
+# --------------------------------------------------------------------------
+# D4 model
+# --------------------------------------------------------------------------
+
+
+def zeta(gam: Tensor, qref: Tensor, qmod: Tensor, ga: float = GA) -> Tensor:
+    """``BaseModel._zeta`` (``model/base.py:308-335``)."""
+    eps = torch.finfo(qmod.dtype).eps
+    scale = torch.exp(gam * (1.0 - qref / (qmod - eps)))
+    return torch.where(
+        qmod > 0.0,
+        torch.exp(ga * (1.0 - scale)),
+        torch.exp(torch.tensor(ga, dtype=qmod.dtype)),
+    )
+
+
+def get_alpha(numbers: Tensor, dtype=torch.float64, gc: float = GC) -> Tensor:
+    """``BaseModel._get_alpha`` (``model/base.py:367-418``), ``eeq``
+    reference charges: (..., n, 7, 23)."""
+    t = tables(dtype)
+    zero = torch.tensor(0.0, dtype=dtype)
+    refsys = t.refsys[numbers]
+    refascale = t.refascale[numbers]
+    refalpha = t.refalpha[numbers]
+    refscount = t.refscount[numbers]
+    refsq = t.clsh[numbers]
+    mask = refsys > 0
+    zeff = t.zeff[refsys]
+    gam = t.gam[refsys] * gc
+    z = torch.where(mask, zeta(gam, zeff, refsq + zeff), zero)
+    aiw = t.secscale[refsys].unsqueeze(-1) * t.secalpha[refsys] * z.unsqueeze(-1)
+    h = refalpha - refscount.unsqueeze(-1) * aiw
+    alpha = refascale.unsqueeze(-1) * h
+    return torch.where(alpha > 0.0, alpha, zero)
+
+
+def trapzd(pol1: Tensor, pol2: Tensor | None = None) -> Tensor:
+    """``utils.trapzd`` (``utils.py:33-94``): Casimir-Polder quadrature."""
+    thopi = 3.0 / 3.141592653589793238462643383279502884197
+    w = tables(pol1.dtype).cp_weights
+    pol2 = pol1 if pol2 is None else pol2
+    return thopi * torch.einsum("w,...iaw,...jbw->...ijab", w, pol1, pol2)
+
+
+def get_refc6(numbers: Tensor, dtype=torch.float64) -> Tensor:
+    """``BaseModel._get_refc6`` (``model/base.py:420-431``)."""
+    return trapzd(get_alpha(numbers, dtype))
+
+
+def weight_references(numbers: Tensor, cn: Tensor | None = None,
+                      q: Tensor | None = None, wf: float = WF,
+                      dtype=torch.float64) -> Tensor:
+    """``D4Model.weight_references`` (``model/d4.py:103-228``), eeq
+    reference charges, no derivatives."""
+    t = tables(dtype)
+    if cn is None:
+        cn = torch.zeros(numbers.shape, dtype=dtype)
+    if q is None:
+        q = torch.zeros(numbers.shape, dtype=dtype)
+    refq = t.clsq[numbers]
+    zero = torch.tensor(0.0, dtype=dtype)
+    zero_double = torch.tensor(0.0, dtype=torch.double)
+    refc = t.refc[numbers]
+    mask = refc > 0
+    refcn = t.refcovcn[numbers].to(torch.double)
+    dcn = cn.unsqueeze(-1).type(torch.double) - refcn
+    tmp = torch.exp(-dcn * dcn)
+
+    def refc_pow(n: int) -> Tensor:
+        return sum(
+            (torch.pow(tmp, i * wf) for i in range(1, n + 1)),
+            torch.tensor(0.0, dtype=torch.double),
+        )
+
+    refc_pow_1 = torch.where(refc == 1, refc_pow(1), tmp)
+    refc_pow_final = torch.where(refc == 3, refc_pow(3), refc_pow_1)
+    expw = torch.where(mask, refc_pow_final, zero_double)
+    norm = torch.where(
+        mask,
+        torch.sum(expw, dim=-1, keepdim=True),
+        torch.tensor(1e-300, dtype=torch.double),
+    )
+    gw_temp = (expw / norm).type(dtype)
+    maxcn = torch.max(refcn, dim=-1, keepdim=True)[0]
+    exceptional = torch.isnan(gw_temp) | (gw_temp > torch.finfo(dtype).max)
+    gw = torch.where(
+        exceptional,
+        torch.where(refcn == maxcn, torch.tensor(1.0, dtype=dtype), zero),
+        gw_temp,
+    )
+    zeff = t.zeff[numbers].unsqueeze(-1)
+    gam = t.gam[numbers].unsqueeze(-1) * GC
+    z = torch.where(mask, zeta(gam, refq + zeff, q.unsqueeze(-1) + zeff), zero)
+    return z * gw
+
+
+def get_atomic_c6(rc6: Tensor, gw: Tensor) -> Tensor:
+    """``D4Model.get_atomic_c6`` (``model/d4.py:268-289``)."""
+    return torch.einsum("...ijab,...ia,...jb->...ij", rc6, gw, gw)
+
+
+def get_polarizabilities(numbers: Tensor, gw: Tensor) -> Tensor:
+    """``BaseModel.get_polarizabilities`` (``model/base.py:286-302``)."""
+    alpha = get_alpha(numbers, gw.dtype)
+    return torch.einsum("...nr,...nr->...n", gw, alpha[..., 0])
+
+
+# --------------------------------------------------------------------------
+# dispersion terms
+# --------------------------------------------------------------------------
+
+
+def rational_damping(distances: Tensor, radii: Tensor, order: int,
+                     a1, a2) -> Tensor:
+    """``RationalDamping._f`` (``damping/functions.py:262-305``)."""
+    radius = a1 * torch.sqrt(radii) + a2
+    return 1.0 / (distances.pow(order) + radius.pow(order))
+
+
+def dispersion2(numbers: Tensor, positions: Tensor, param: dict, c6: Tensor,
+                r4r2: Tensor, cutoff: float = D4_DISP2_CUTOFF,
+                as_matrix: bool = False) -> Tensor:
+    """``dispersion2`` (``dispersion/twobody.py:89-201``) with rational
+    damping. The order-8/10 terms follow the key-presence rules of
+    ``Damping.__call__`` (``damping/functions.py:141-152``)."""
+    dtype = positions.dtype
+    zero = torch.tensor(0.0, dtype=dtype)
+    mask = real_pairs(numbers, mask_diagonal=True)
+    distances = torch.where(mask, cdist(positions),
+                            torch.tensor(torch.finfo(dtype).eps, dtype=dtype))
+    qq = 3 * r4r2.unsqueeze(-1) * r4r2.unsqueeze(-2)
+    c8 = c6 * qq
+    a1, a2 = param["a1"], param["a2"]
+    inside = mask * (distances <= cutoff)
+
+    def damp(order: int) -> Tensor:
+        return torch.where(inside, rational_damping(distances, qq, order, a1, a2),
+                           zero)
+
+    t6 = damp(6)
+    has8 = param.get("s8") is not None or param.get("rs8") is not None
+    t8 = damp(8) if has8 else torch.zeros_like(distances)
+    if as_matrix:
+        e6, e8 = c6 * t6, c8 * t8
+    else:
+        e6 = torch.sum(c6 * t6, dim=-1)
+        e8 = torch.sum(c8 * t8, dim=-1)
+    s6 = param.get("s6", torch.tensor(S6, dtype=dtype))
+    s8 = param.get("s8", torch.tensor(S8, dtype=dtype))
+    edisp = s6 * e6 + s8 * e8
+    if "s10" in param:
+        c10 = c6 * torch.pow(qq, 2) * 49.0 / 40.0
+        t10 = damp(10) if param["s10"] is not None else torch.zeros_like(distances)
+        e10 = c10 * t10 if as_matrix else torch.sum(c10 * t10, dim=-1)
+        edisp = edisp + param["s10"] * e10
+    if as_matrix:
+        return -edisp
+    return -0.5 * edisp
+
+
+def zero_damping_order9(distances: Tensor, radii: Tensor, alp) -> Tensor:
+    """``ZeroDamping._f`` with order 9 and ``only_damping=True``
+    (``damping/functions.py:329-378``), rs9 = 1 (``defaults.py:83``)."""
+    t_n = RS9 * safe_divide(distances, radii)
+    return 1.0 / (1.0 + 6.0 * t_n**alp)
+
+
+def atm_dispersion(numbers: Tensor, positions: Tensor, c9: Tensor,
+                   radii: Tensor, cutoff: float, s9, alp) -> Tensor:
+    """``get_atm_dispersion`` (``dispersion/threebody.py:54-163``). Only the
+    edges ij and jk are screened (``threebody.py:153-160``)."""
+    dtype = positions.dtype
+    cutoff2 = cutoff * cutoff
+    mask_pairs = real_pairs(numbers, mask_diagonal=True)
+    mask_triples = real_triples(numbers)
+    eps = torch.tensor(torch.finfo(dtype).eps, dtype=dtype)
+    zero = torch.tensor(0.0, dtype=dtype)
+    one = torch.tensor(1.0, dtype=dtype)
+
+    r0ij = radii.unsqueeze(-1)
+    r0ik = radii.unsqueeze(-2)
+    r0jk = radii.unsqueeze(-3)
+    r0 = r0ij * r0ik * r0jk
+
+    distances = torch.pow(torch.where(mask_pairs, cdist(positions), eps), 2.0)
+    r2ij = distances.unsqueeze(-1)
+    r2ik = distances.unsqueeze(-2)
+    r2jk = distances.unsqueeze(-3)
+    r2 = r2ij * r2ik * r2jk
+    r1 = torch.sqrt(r2)
+    r3 = torch.where(mask_triples, r1 * r2, eps)
+    r5 = torch.where(mask_triples, r2 * r3, eps)
+
+    fdamp = torch.where(
+        mask_triples,
+        zero_damping_order9(r0, torch.where(mask_triples, r1, one), alp / 3.0),
+        zero,
+    )
+    s = torch.where(
+        mask_triples,
+        (r2ij + r2jk - r2ik) * (r2ij - r2jk + r2ik) * (-r2ij + r2jk + r2ik),
+        zero,
+    )
+    ang = torch.where(
+        mask_triples * (r2ij <= cutoff2) * (r2jk <= cutoff2),
+        0.375 * s / r5 + 1.0 / r3,
+        zero,
+    )
+    energy = ang * fdamp * s9 * c9
+    return torch.sum(energy, dim=(-2, -1)) / 6.0
+
+
+def atm_term(numbers: Tensor, positions: Tensor, param: dict, cn: Tensor,
+             rc6: Tensor, r4r2: Tensor, cutoff: float) -> Tensor:
+    """``D4ATMApprox.calculate``: ``C9ApproxMixin.get_c9`` with q = None
+    (``threebody.py:311-321``), ``RadiiBJMixin.get_radii``
+    (``threebody.py:244-256``), then ``get_atm_dispersion``."""
+    dtype = positions.dtype
+    gw = weight_references(numbers, cn, None, dtype=dtype)
+    c6 = get_atomic_c6(rc6, gw)
+    c9 = safe_sqrt(torch.abs(c6.unsqueeze(-1) * c6.unsqueeze(-2) * c6.unsqueeze(-3)))
+    a1 = param.get("a1", torch.tensor(A1, dtype=dtype))
+    a2 = param.get("a2", torch.tensor(A2, dtype=dtype))
+    radii = a1 * safe_sqrt(3.0 * r4r2.unsqueeze(-1) * r4r2.unsqueeze(-2)) + a2
+    s9 = param.get("s9", torch.tensor(S9, dtype=dtype))
+    alp = param.get("alp", torch.tensor(ALP, dtype=dtype))
+    return atm_dispersion(numbers, positions, c9, radii, cutoff, s9, alp)
+
+
+def dftd4(numbers: Tensor, positions: Tensor, charge, param: dict,
+          q: Tensor | None = None, cutoff: dict | None = None,
+          return_parts: bool = False):
+    """``dftd4`` (``disp.py:44-146``) -> ``Disp.calculate``
+    (``dispersion/base.py:285-431``): CN, EEQ charges, two-body term with
+    charge-dependent C6 and the approximate-C9 ATM term.
+
+    ``cutoff`` keys: ``disp2``, ``disp3``, ``cn_eeq`` (``cutoff.py:36-90``;
+    ``cn`` is accepted and ignored, as in the reference)."""
+    dtype = positions.dtype
+    c = {"disp2": D4_DISP2_CUTOFF, "disp3": D4_DISP3_CUTOFF,
+         "cn_eeq": D4_CN_EEQ_CUTOFF}
+    if cutoff:
+        c.update({k: float(v) for k, v in cutoff.items() if k in c})
+    charge = torch.as_tensor(charge, dtype=dtype)
+    if numbers.shape != positions.shape[:-1]:
+        raise ValueError("Shape of positions is not consistent with numbers.")
+    t = tables(dtype)
+    r4r2 = t.r4r2[numbers]
+    cn = cn_d4(numbers, positions)
+    if q is None:
+        q = eeq_charges(numbers, positions, charge, cutoff=c["cn_eeq"])
+    rc6 = get_refc6(numbers, dtype)
+    gw = weight_references(numbers, cn, q, dtype=dtype)
+    c6 = get_atomic_c6(rc6, gw)
+    e2 = dispersion2(numbers, positions, param, c6, r4r2, cutoff=c["disp2"])
+    e3 = atm_term(numbers, positions, param, cn, rc6, r4r2, c["disp3"])
+    if return_parts:
+        return {"e2": e2, "e3": e3, "cn": cn, "q": q, "c6": c6, "gw": gw}
+    return e2 + e3
+
+
+def dftd4_with_grad(numbers: Tensor, positions: Tensor, charge, param: dict,
+                    cutoff: dict | None = None) -> tuple[Tensor, Tensor]:
+    """Atom energies and the autograd gradient dE/dR of their sum
+    (the reference's force path, ``examples/forces.py:47-50``)."""
+    pos = positions.detach().clone().requires_grad_(True)
+    e = dftd4(numbers, pos, charge, param, cutoff=cutoff)
+    (g,) = torch.autograd.grad(e.sum(), pos)
+    return e.detach(), g

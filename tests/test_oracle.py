@@ -1,0 +1,304 @@
+"""
+Pins the CPU oracle (``oracle.dense``) before it is trusted as the parity
+checker for the HIP kernels.
+
+Each check is chosen so that a plausible mistake fails it: golden vectors
+from the reference (end-to-end energies, tblite gradients, C6 from fixed
+weights), an independent loop formulation (``oracle.brute``), closed forms,
+invariants and finite differences.
+"""
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import brute, dense
+from tad_dftd4_amd import synth
+from tad_dftd4_amd.data import elements
+
+GOLD = json.loads(
+    (Path(__file__).parent / "golden" / "reference_vectors.json").read_text()
+)
+F64 = torch.float64
+
+
+def _p(d):
+    return {k: torch.tensor(v, dtype=F64) for k, v in d.items()}
+
+
+def _mol(name):
+    g = GOLD[name]
+    return torch.tensor(g["numbers"]), torch.tensor(g["positions"], dtype=F64)
+
+
+# ---------------------------------------------------------------- golden
+
+
+def test_example_single_energies():
+    """examples/single.py:52-67 (C, N, S, H). The reference ran in float32
+    and printed 10 decimals, so the bound is 1e-9 Eh."""
+    g = GOLD["example_single"]
+    e = dense.dftd4(torch.tensor(g["numbers"]), torch.tensor(g["positions"], dtype=F64),
+                    0.0, _p(g["param"]))
+    ref = torch.tensor(g["energy"], dtype=F64)
+    assert (e - ref).abs().max().item() < 1e-9
+
+
+def test_example_batch_totals():
+    """Formamide dimer / formamide totals (src/tad_dftd4/__init__.py:84-87),
+    padded batch (C, N, O, H). float32 reference: bound 2e-9 Eh."""
+    g = GOLD["example_batch"]
+    mols = [synth.Molecule(np.array(n), np.array(p), 0.0)
+            for n, p in zip(g["numbers"], g["positions"])]
+    numbers, positions, charge = synth.pack(mols)
+    e = dense.dftd4(numbers, positions, charge, _p(g["param"])).sum(-1)
+    ref = torch.tensor(g["total_energy"], dtype=F64)
+    assert (e - ref).abs().max().item() < 2e-9
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_tblite_gradient(name):
+    """tblite gradients (test/test_grad/samples_grad.py) with TPSS0-ATM
+    parameters; covers CN, EEQ (dq/dR), weights, C6, E2 and ATM."""
+    numbers, positions = _mol(name)
+    _, g = dense.dftd4_with_grad(numbers, positions, 0.0, _p(GOLD["param_tpss0"]))
+    ref = torch.tensor(GOLD[name]["grad"], dtype=F64)
+    assert (g - ref).abs().max().item() < 1e-13
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_fortran_energies_with_given_charges(name):
+    """test/test_d4/samples.py disp/disp3 with the fixture charges, the same
+    call as test/test_d4/test_full.py:58-84 (reference tolerance 1.5e-7)."""
+    numbers, positions = _mol(name)
+    q = torch.tensor(GOLD[name]["q"], dtype=F64)
+    parts = dense.dftd4(numbers, positions, 0.0, _p(GOLD["param_tpssh"]), q=q,
+                        return_parts=True)
+    disp = torch.tensor(GOLD[name]["disp"], dtype=F64)
+    disp3 = torch.tensor(GOLD[name]["disp3"], dtype=F64)
+    assert ((parts["e2"] + parts["e3"]) - disp).abs().max().item() < 1e-10
+    assert (parts["e3"] - disp3).abs().max().item() < 1e-15
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_eeq_charges_fixture(name):
+    """EEQ charges vs the dftd4 test-suite charges. Those were produced by an
+    older EEQ build; the reference itself only agrees with them to its
+    1.5e-7 test tolerance (test/test_d4/test_twobody.py:154-168)."""
+    numbers, positions = _mol(name)
+    q = dense.eeq_charges(numbers, positions, torch.tensor(0.0, dtype=F64))
+    ref = torch.tensor(GOLD[name]["q"], dtype=F64)
+    assert (q - ref).abs().max().item() < 5e-7
+    assert abs(q.sum().item()) < 1e-14
+
+
+def _fixture_gw(name):
+    g = GOLD[name]["gw"]
+    nref, nat = g["fortran_shape"]
+    gw = torch.tensor(g["values"], dtype=F64).reshape(nat, nref)
+    return torch.nn.functional.pad(gw, (0, 7 - nref))
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_c6_from_fixture_weights(name):
+    """test/test_model/test_c6.py:36-58: C6 from fixed weights pins the
+    reference C6 table (alpha, zeta of secondary systems, quadrature)."""
+    numbers, _ = _mol(name)
+    rc6 = dense.get_refc6(numbers)
+    c6 = dense.get_atomic_c6(rc6, _fixture_gw(name))
+    n = len(numbers)
+    ref = torch.tensor(GOLD[name]["c6"], dtype=F64).reshape(n, n)
+    assert torch.allclose(c6, ref, rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_weights_fixture_and_normalisation(name):
+    """Weights from the fixture geometry and charges agree with the fixture
+    weights to the reference's own tolerance (test_model/test_weights.py:45;
+    the Fortran fixture differs by ~1.4e-7 relative), and
+    sum_r gw_r / zeta_r = 1 (normalisation, model/d4.py:200)."""
+    numbers, positions = _mol(name)
+    q = torch.tensor(GOLD[name]["q"], dtype=F64)
+    cn = dense.cn_d4(numbers, positions)
+    gw = dense.weight_references(numbers, cn, q)
+    assert torch.allclose(gw, _fixture_gw(name), rtol=1e-6, atol=1e-12)
+    t = dense.tables()
+    refc = t.refc[numbers]
+    zeff = t.zeff[numbers].unsqueeze(-1)
+    z = dense.zeta(t.gam[numbers].unsqueeze(-1) * dense.GC,
+                   t.clsq[numbers] + zeff, q.unsqueeze(-1) + zeff)
+    s = torch.where(refc > 0, gw / z, torch.zeros_like(gw)).sum(-1)
+    assert torch.allclose(s, torch.ones_like(s), atol=1e-14)
+
+
+# ------------------------------------------------------------- brute force
+
+
+def _rand_small(seed, n, spacing=2.6, mix=synth.MIX_Z54, charges=(-1, 0, 1)):
+    rng = np.random.default_rng(seed)
+    return synth.molecule(rng, n, mix, spacing, charges)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_brute_cn_eeq_e2_e3(seed):
+    m = _rand_small(seed, 7)
+    numbers = torch.tensor(m.numbers)
+    positions = torch.tensor(m.positions)
+    z, p = m.numbers.tolist(), m.positions.tolist()
+    rcov, en = elements.COV_D3, elements.PAULING_EN
+    cn = dense.cn_d4(numbers, positions)
+    assert np.allclose(cn.numpy(), brute.cn_d4(z, p, rcov, en), rtol=1e-13, atol=0)
+    q = dense.eeq_charges(numbers, positions, torch.tensor(m.charge, dtype=F64))
+    qb = brute.eeq_charges(z, p, m.charge, rcov, elements.EEQ_CHI, elements.EEQ_ETA,
+                           elements.EEQ_KCN, elements.EEQ_RAD)
+    assert np.allclose(q.numpy(), qb, rtol=0, atol=1e-12)
+    par = _p({"s6": 1.0, "s8": 1.2, "s9": 1.0, "a1": 0.4, "a2": 5.0})
+    # a cutoff that splits the molecule so the screening rules are exercised
+    cut = {"disp2": 6.0, "disp3": 5.0, "cn_eeq": 25.0}
+    parts = dense.dftd4(numbers, positions, m.charge, par, cutoff=cut,
+                        return_parts=True)
+    r4r2 = dense.tables().r4r2.tolist()
+    e2 = brute.twobody(z, p, parts["c6"].tolist(), r4r2, 1.0, 1.2, 0.4, 5.0, 6.0)
+    assert np.allclose(parts["e2"].numpy(), e2, rtol=1e-13, atol=1e-18)
+    rc6 = dense.get_refc6(numbers)
+    c60 = dense.get_atomic_c6(rc6, dense.weight_references(numbers, cn, None))
+    e3 = brute.threebody(z, p, c60.tolist(), r4r2, 1.0, 0.4, 5.0, 16.0, 5.0)
+    assert np.allclose(parts["e3"].numpy(), e3, rtol=1e-12, atol=1e-20)
+
+
+def test_brute_atm_screening_is_asymmetric():
+    """A triple whose edge ik alone exceeds the cutoff still contributes to
+    atom i (threebody.py:153-160 screens only ij and jk)."""
+    numbers = torch.tensor([6, 6, 6])
+    positions = torch.tensor([[0.0, 0, 0], [3.0, 0, 0], [6.0, 0, 0]], dtype=F64)
+    cn = dense.cn_d4(numbers, positions)
+    rc6 = dense.get_refc6(numbers)
+    c60 = dense.get_atomic_c6(rc6, dense.weight_references(numbers, cn, None))
+    r4r2 = dense.tables().r4r2
+    radii = 0.4 * torch.sqrt(3.0 * r4r2[numbers].unsqueeze(-1) * r4r2[numbers]) + 5.0
+    c9 = torch.sqrt(torch.abs(c60.unsqueeze(-1) * c60.unsqueeze(-2) * c60.unsqueeze(-3)))
+    e = dense.atm_dispersion(numbers, positions, c9, radii, 4.0, 1.0, 16.0)
+    # edges 0-1 and 1-2 are inside 4 bohr, 0-2 is not: atoms 0 and 2 each get
+    # e/6 from one ordered pair, the middle atom gets nothing.
+    assert e[0].item() != 0.0 and e[2].item() != 0.0
+    assert e[1].item() == 0.0
+    eb = brute.threebody([6, 6, 6], positions.tolist(), c60.tolist(),
+                         r4r2.tolist(), 1.0, 0.4, 5.0, 16.0, 4.0)
+    assert np.allclose(e.numpy(), eb, rtol=1e-13)
+
+
+# --------------------------------------------------------- closed forms
+
+
+def test_two_atom_closed_forms():
+    """Two atoms: CN = den * erfcount, EEQ solved by hand, ATM = 0."""
+    numbers = torch.tensor([8, 1])
+    r = 1.9
+    positions = torch.tensor([[0.0, 0, 0], [0.0, 0, r]], dtype=F64)
+    t = dense.tables()
+    rc = t.cov_d3[8] + t.cov_d3[1]
+    den = dense.D4_K4 * math.exp(-((abs(t.en[8] - t.en[1]).item() + dense.D4_K5) ** 2)
+                                 / dense.D4_K6)
+    cnt = 0.5 * (1 + math.erf(-7.5 * (r / rc.item() - 1)))
+    cn = dense.cn_d4(numbers, positions)
+    assert math.isclose(cn[0].item(), den * cnt, rel_tol=1e-14)
+    # EEQ: [[a b 1][b c 1][1 1 0]] x = [u v Q] -> q0 = (u - v + Q(c - b)) / (a + c - 2b)
+    cap = math.log(1 + math.exp(8)) - math.log(1 + math.exp(8 - cnt))
+    a = t.eta[8].item() + math.sqrt(2 / math.pi) / t.rad[8].item()
+    c = t.eta[1].item() + math.sqrt(2 / math.pi) / t.rad[1].item()
+    g = 1 / math.sqrt(t.rad[8].item() ** 2 + t.rad[1].item() ** 2)
+    b = math.erf(g * r) / r
+    u = -t.chi[8].item() + t.kcn[8].item() * math.sqrt(cap)
+    v = -t.chi[1].item() + t.kcn[1].item() * math.sqrt(cap)
+    q0 = (u - v) / (a + c - 2 * b)
+    q = dense.eeq_charges(numbers, positions, torch.tensor(0.0, dtype=F64))
+    assert math.isclose(q[0].item(), q0, rel_tol=1e-13)
+    par = _p({"s6": 1.0, "s8": 1.2, "s9": 1.0, "a1": 0.4, "a2": 5.0})
+    parts = dense.dftd4(numbers, positions, 0.0, par, return_parts=True)
+    assert torch.all(parts["e3"] == 0)
+
+
+def test_trapzd_is_explicit_sum():
+    numbers = torch.tensor([6, 1])
+    alpha = dense.get_alpha(numbers)
+    w = dense.tables().cp_weights
+    c = dense.trapzd(alpha)
+    manual = 3.0 / math.pi * sum(w[k] * alpha[0, 2, k] * alpha[1, 1, k] for k in range(23))
+    assert math.isclose(c[0, 1, 2, 1].item(), manual.item(), rel_tol=1e-14)
+
+
+def test_c8_and_as_matrix():
+    """E = 1/2 sum of the as_matrix form (test_d4/test_twobody.py:113-141)."""
+    m = _rand_small(11, 9)
+    numbers, positions = torch.tensor(m.numbers), torch.tensor(m.positions)
+    par = _p({"s6": 1.0, "s8": 1.2, "a1": 0.4, "a2": 5.0})
+    cn = dense.cn_d4(numbers, positions)
+    c6 = dense.get_atomic_c6(dense.get_refc6(numbers),
+                             dense.weight_references(numbers, cn, None))
+    r4r2 = dense.tables().r4r2[numbers]
+    e = dense.dispersion2(numbers, positions, par, c6, r4r2)
+    em = dense.dispersion2(numbers, positions, par, c6, r4r2, as_matrix=True)
+    assert torch.allclose(e, 0.5 * em.sum(-1), rtol=1e-14)
+
+
+# ------------------------------------------------------------- invariants
+
+
+def test_invariants_translation_rotation_forces():
+    m = _rand_small(5, 10)
+    numbers, positions = torch.tensor(m.numbers), torch.tensor(m.positions)
+    par = _p({"s6": 1.0, "s8": 1.2, "s9": 1.0, "a1": 0.4, "a2": 5.0})
+    e, g = dense.dftd4_with_grad(numbers, positions, m.charge, par)
+    assert abs(g.sum(0)).max().item() < 1e-15 * 1e3
+    rot = torch.tensor(
+        [[0.36, 0.48, -0.8], [-0.8, 0.6, 0.0], [0.48, 0.64, 0.6]], dtype=F64
+    )
+    e2 = dense.dftd4(numbers, positions @ rot.T + 3.7, m.charge, par)
+    assert torch.allclose(e, e2, rtol=1e-12, atol=1e-18)
+
+
+def test_charge_conservation_and_padding():
+    mols = [_rand_small(s, n) for s, n in ((1, 6), (2, 11), (3, 3))]
+    numbers, positions, charge = synth.pack(mols, pad_to=13)
+    q = dense.eeq_charges(numbers, positions, charge)
+    assert torch.allclose(q.sum(-1), charge, atol=1e-13)
+    par = _p({"s6": 1.0, "s8": 1.2, "s9": 1.0, "a1": 0.4, "a2": 5.0})
+    e = dense.dftd4(numbers, positions, charge, par)
+    for k, m in enumerate(mols):
+        n = len(m.numbers)
+        es = dense.dftd4(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge, par)
+        assert torch.allclose(e[k, :n], es, rtol=1e-12, atol=1e-18)
+        assert torch.all(e[k, n:] == 0)
+
+
+def test_term_switches():
+    """s9 = 0 leaves the two-body term; s6 = s8 = 0 leaves ATM
+    (test_d4/test_twobody.py:144-168, test_d4/test_atm.py:143-169)."""
+    m = _rand_small(7, 8)
+    numbers, positions = torch.tensor(m.numbers), torch.tensor(m.positions)
+    par = {"s6": 1.0, "s8": 1.2, "s9": 1.0, "a1": 0.4, "a2": 5.0}
+    parts = dense.dftd4(numbers, positions, m.charge, _p(par), return_parts=True)
+    e_no3 = dense.dftd4(numbers, positions, m.charge, _p({**par, "s9": 0.0}))
+    e_only3 = dense.dftd4(numbers, positions, m.charge, _p({**par, "s6": 0.0, "s8": 0.0}))
+    assert torch.allclose(e_no3, parts["e2"], rtol=1e-14)
+    assert torch.allclose(e_only3, parts["e3"], rtol=1e-14, atol=1e-25)
+
+
+def test_finite_difference_gradient():
+    """Central differences, h = 1e-5 bohr (examples/forces.py:53-73)."""
+    m = _rand_small(9, 8)
+    numbers, positions = torch.tensor(m.numbers), torch.tensor(m.positions)
+    par = _p({"s6": 1.0, "s8": 1.2, "s9": 1.0, "a1": 0.4, "a2": 5.0})
+    _, g = dense.dftd4_with_grad(numbers, positions, m.charge, par)
+    h = 1e-5
+    for i, x in ((0, 0), (3, 1), (7, 2)):
+        pp, pm = positions.clone(), positions.clone()
+        pp[i, x] += h
+        pm[i, x] -= h
+        fd = (dense.dftd4(numbers, pp, m.charge, par).sum()
+              - dense.dftd4(numbers, pm, m.charge, par).sum()) / (2 * h)
+        assert abs(fd.item() - g[i, x].item()) < 1e-10
