@@ -1,0 +1,172 @@
+/*
+ * d4amd: C ABI of the MI355X (gfx950) DFT-D4 energy and gradient kernels.
+ *
+ * Plain pointers and sizes only. Every device pointer argument is memory
+ * that the caller owns (allocated by the caller, e.g. by PyTorch); the
+ * library owns only its per-device context (the element tables resident in
+ * HBM). Streams are passed as `void*` holding a `hipStream_t`; every entry
+ * point only enqueues work on that stream (no host synchronisation), so a
+ * caller may capture the calls in a HIP graph.
+ *
+ * Batch convention (as in the reference): `numbers` is (nbatch, natoms)
+ * int64 with Z = 0 marking padding atoms, `positions` is (nbatch, natoms, 3)
+ * float64 in bohr, `charge` is (nbatch,) float64. Outputs for padding atoms
+ * are zero.
+ *
+ * Errors: every function returns D4_OK (0) or a negative D4_ERR_* code and
+ * never throws; d4_last_error() returns a thread-local message.
+ */
+#ifndef D4AMD_H
+#define D4AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D4_OK 0
+#define D4_ERR_INVALID (-1)   /* bad argument: null pointer, size, Z range */
+#define D4_ERR_DEVICE (-2)    /* HIP runtime error */
+#define D4_ERR_NOMEM (-3)     /* device allocation failed */
+#define D4_ERR_WORKSPACE (-4) /* workspace smaller than d4_workspace_bytes */
+
+#define D4_MAX_Z 104   /* element slots 0..103 (0 = padding) */
+#define D4_MAX_REF 7   /* reference systems per element */
+
+/* d4_params.flags */
+#define D4_FLAG_S10 1      /* "s10" key present: evaluate the order-10 term */
+#define D4_FLAG_FORCE_LU 2 /* testing: solve every EEQ system by pivoted LU */
+
+/* Element and reference-system tables, host memory, read once by
+ * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference
+ * arrays have D4_MAX_Z * D4_MAX_REF entries with the used reference systems
+ * of each element compacted to the front (nref[z] of them). */
+typedef struct d4_tables {
+  const double* rcov;  /* covalent radii (bohr) for the CN counting functions */
+  const double* en;    /* Pauling electronegativities */
+  const double* r4r2;  /* sqrt(1/2 <r4>/<r2> sqrt(Z)) */
+  const double* eeq_chi;
+  const double* eeq_eta;
+  const double* eeq_kcn;
+  const double* eeq_rad;
+  const double* gam;   /* chemical hardness (multiplied by gc in the kernels) */
+  const double* zeff;  /* effective nuclear charge */
+  const double* maxcn; /* max reference CN over all 7 slots (exceptional weights) */
+  const int32_t* nref; /* used reference systems per element */
+  const int32_t* ngw;  /* Gaussians per reference (1 or 3) */
+  const double* refcn; /* reference coordination numbers */
+  const double* refq;  /* reference charges */
+  const double* alpha0; /* static reference polarisabilities alpha(i*0) */
+  const double* rc6;   /* [zi][zj][a][b] reference C6, D4_MAX_Z^2 * 49 */
+} d4_tables;
+
+/* Damping parameters, model constants and cutoffs. Absent optional terms
+ * are passed as zero (s8: order-8 term off; s10 with D4_FLAG_S10 clear). */
+typedef struct d4_params {
+  double s6, s8, s10, s9;
+  double a1, a2, alp;
+  double ga, gc, wf;
+  double cutoff_disp2; /* two-body cutoff (bohr) */
+  double cutoff_disp3; /* three-body cutoff (bohr) */
+  double cutoff_cn;    /* D4 coordination number cutoff (bohr) */
+  double cutoff_eeq;   /* EEQ coordination number cutoff (bohr) */
+  int32_t flags;
+  int32_t reserved;
+} d4_params;
+
+typedef struct d4_context d4_context;
+
+/* Create a context on `device` and upload the tables to its HBM.
+ * Replaces the per-call table build of D4Model (reference
+ * src/tad_dftd4/model/base.py:107-151, model/base.py:367-431). */
+int d4_context_create(int device, const d4_tables* tables, d4_context** out);
+
+/* Release the context and its device tables. */
+int d4_context_destroy(d4_context* ctx);
+
+/* Bytes of device workspace needed for a batch of nbatch molecules padded to
+ * natoms atoms (forward, backward and the fused energy+gradient path share
+ * one layout; the forward leaves in it what the backward reads). */
+int d4_workspace_bytes(int64_t nbatch, int64_t natoms, size_t* bytes);
+
+/* Atom-resolved dispersion energy, energy_out (nbatch, natoms).
+ * Replaces dftd4() (reference src/tad_dftd4/disp.py:44-146) and
+ * Disp.calculate (src/tad_dftd4/dispersion/base.py:285-431).
+ * q_in: optional (nbatch, natoms) charges (NULL: EEQ charges, as
+ *   dispersion/base.py:401-407); r4r2_in: optional per-atom r4r2 values
+ *   (NULL: element table, as dispersion/base.py:373-374).
+ * q_out: optional, receives the charges used. */
+int d4_forward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+               const int64_t* numbers, const double* positions,
+               const double* charge, const double* q_in, const double* r4r2_in,
+               const d4_params* params, double* energy_out, double* q_out,
+               void* workspace, size_t workspace_bytes);
+
+/* Gradient of sum_i grad_energy[i] * E_i with respect to the positions,
+ * written to grad_positions (nbatch, natoms, 3). Must follow d4_forward with
+ * the same inputs and workspace. grad_energy NULL means all ones.
+ * Replaces the autograd backward of dftd4() (reference
+ * test/test_grad/test_pos.py:185-187). */
+int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                const int64_t* numbers, const double* positions,
+                const double* charge, const double* q_in,
+                const double* r4r2_in, const d4_params* params,
+                const double* grad_energy, double* grad_positions,
+                void* workspace, size_t workspace_bytes);
+
+/* Fused atom energies and the gradient of the total energy in one pass.
+ * Replaces dftd4() followed by torch.autograd.grad(energy.sum(), positions)
+ * (reference examples/forces.py:47-50). */
+int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch,
+                       int64_t natoms, const int64_t* numbers,
+                       const double* positions, const double* charge,
+                       const double* q_in, const double* r4r2_in,
+                       const d4_params* params, double* energy_out,
+                       double* grad_positions, void* workspace,
+                       size_t workspace_bytes);
+
+/* Per-atom properties: D4 coordination numbers, charges, the C6 matrix
+ * (nbatch, natoms, natoms) and static polarisabilities (nbatch, natoms).
+ * Replaces get_properties() (reference src/tad_dftd4/disp.py:149-197).
+ * Any output pointer may be NULL. */
+int d4_properties(d4_context* ctx, void* stream, int64_t nbatch,
+                  int64_t natoms, const int64_t* numbers,
+                  const double* positions, const double* charge,
+                  const d4_params* params, double* cn_out, double* q_out,
+                  double* c6_out, double* alpha_out, void* workspace,
+                  size_t workspace_bytes);
+
+/* Per-molecule status words of the last call that used `workspace`, copied
+ * to host memory (synchronises the stream). Bit 0: EEQ solved by the
+ * pivoted-LU path; bit 1: EEQ system singular; bit 2: Z out of range. */
+int d4_molecule_status(void* stream, int64_t nbatch, const void* workspace,
+                       int32_t* status_out);
+
+/* Diagnostics (no counterpart in the reference): per-kernel HIP-event
+ * timing on the launch stream. While enabled, every launch made through
+ * `ctx` is bracketed by events; d4_profile_read synchronises the context's
+ * last stream, adds the elapsed times up per kernel slot (D4_SLOT_*) and
+ * resets the accumulators. */
+#define D4_SLOT_PREP 0
+#define D4_SLOT_CN 1
+#define D4_SLOT_EEQ 2
+#define D4_SLOT_WEIGHTS 3
+#define D4_SLOT_PAIRS 4
+#define D4_SLOT_ATM 5
+#define D4_SLOT_ATM_ROWS 6
+#define D4_SLOT_ADJOINT 7
+#define D4_SLOT_CHAIN 8
+#define D4_NUM_SLOTS 9
+int d4_profile_enable(d4_context* ctx, int on);
+int d4_profile_read(d4_context* ctx, double* ms_out, int64_t* count_out, int nslots);
+
+/* Thread-local message describing the last error. */
+const char* d4_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* D4AMD_H */

@@ -307,9 +307,15 @@ def get_refc6(numbers: Tensor, dtype=torch.float64) -> Tensor:
 
 def weight_references(numbers: Tensor, cn: Tensor | None = None,
                       q: Tensor | None = None, wf: float = WF,
-                      dtype=torch.float64) -> Tensor:
+                      dtype=torch.float64, nan_safe: bool = False) -> Tensor:
     """``D4Model.weight_references`` (``model/d4.py:103-228``), eeq
-    reference charges, no derivatives."""
+    reference charges, no derivatives.
+
+    With ``nan_safe`` the values are unchanged but the division whose result
+    the exceptional fallback discards (all Gaussians underflow, norm = 0)
+    gets a unit denominator, so that autograd does not propagate the 0/0 of
+    the unused branch. The reference (``nan_safe=False``) returns NaN
+    gradients for such atoms."""
     t = tables(dtype)
     if cn is None:
         cn = torch.zeros(numbers.shape, dtype=dtype)
@@ -341,6 +347,9 @@ def weight_references(numbers: Tensor, cn: Tensor | None = None,
     gw_temp = (expw / norm).type(dtype)
     maxcn = torch.max(refcn, dim=-1, keepdim=True)[0]
     exceptional = torch.isnan(gw_temp) | (gw_temp > torch.finfo(dtype).max)
+    if nan_safe:
+        safe_norm = torch.where(exceptional, torch.ones_like(norm), norm)
+        gw_temp = (expw / safe_norm).type(dtype)
     gw = torch.where(
         exceptional,
         torch.where(refcn == maxcn, torch.tensor(1.0, dtype=dtype), zero),
@@ -469,12 +478,12 @@ def atm_dispersion(numbers: Tensor, positions: Tensor, c9: Tensor,
 
 
 def atm_term(numbers: Tensor, positions: Tensor, param: dict, cn: Tensor,
-             rc6: Tensor, r4r2: Tensor, cutoff: float) -> Tensor:
+             rc6: Tensor, r4r2: Tensor, cutoff: float, nan_safe: bool = False) -> Tensor:
     """``D4ATMApprox.calculate``: ``C9ApproxMixin.get_c9`` with q = None
     (``threebody.py:311-321``), ``RadiiBJMixin.get_radii``
     (``threebody.py:244-256``), then ``get_atm_dispersion``."""
     dtype = positions.dtype
-    gw = weight_references(numbers, cn, None, dtype=dtype)
+    gw = weight_references(numbers, cn, None, dtype=dtype, nan_safe=nan_safe)
     c6 = get_atomic_c6(rc6, gw)
     c9 = safe_sqrt(torch.abs(c6.unsqueeze(-1) * c6.unsqueeze(-2) * c6.unsqueeze(-3)))
     a1 = param.get("a1", torch.tensor(A1, dtype=dtype))
@@ -487,7 +496,7 @@ def atm_term(numbers: Tensor, positions: Tensor, param: dict, cn: Tensor,
 
 def dftd4(numbers: Tensor, positions: Tensor, charge, param: dict,
           q: Tensor | None = None, cutoff: dict | None = None,
-          return_parts: bool = False):
+          return_parts: bool = False, nan_safe: bool = False):
     """``dftd4`` (``disp.py:44-146``) -> ``Disp.calculate``
     (``dispersion/base.py:285-431``): CN, EEQ charges, two-body term with
     charge-dependent C6 and the approximate-C9 ATM term.
@@ -508,20 +517,23 @@ def dftd4(numbers: Tensor, positions: Tensor, charge, param: dict,
     if q is None:
         q = eeq_charges(numbers, positions, charge, cutoff=c["cn_eeq"])
     rc6 = get_refc6(numbers, dtype)
-    gw = weight_references(numbers, cn, q, dtype=dtype)
+    gw = weight_references(numbers, cn, q, dtype=dtype, nan_safe=nan_safe)
     c6 = get_atomic_c6(rc6, gw)
     e2 = dispersion2(numbers, positions, param, c6, r4r2, cutoff=c["disp2"])
-    e3 = atm_term(numbers, positions, param, cn, rc6, r4r2, c["disp3"])
+    e3 = atm_term(numbers, positions, param, cn, rc6, r4r2, c["disp3"], nan_safe)
     if return_parts:
         return {"e2": e2, "e3": e3, "cn": cn, "q": q, "c6": c6, "gw": gw}
     return e2 + e3
 
 
 def dftd4_with_grad(numbers: Tensor, positions: Tensor, charge, param: dict,
-                    cutoff: dict | None = None) -> tuple[Tensor, Tensor]:
+                    cutoff: dict | None = None, nan_safe: bool = True
+                    ) -> tuple[Tensor, Tensor]:
     """Atom energies and the autograd gradient dE/dR of their sum
-    (the reference's force path, ``examples/forces.py:47-50``)."""
+    (the reference's force path, ``examples/forces.py:47-50``). By default
+    the exceptional-weight branch is made NaN-safe (see
+    ``weight_references``); energies are identical either way."""
     pos = positions.detach().clone().requires_grad_(True)
-    e = dftd4(numbers, pos, charge, param, cutoff=cutoff)
+    e = dftd4(numbers, pos, charge, param, cutoff=cutoff, nan_safe=nan_safe)
     (g,) = torch.autograd.grad(e.sum(), pos)
     return e.detach(), g

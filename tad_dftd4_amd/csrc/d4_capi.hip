@@ -1,0 +1,376 @@
+// C ABI of the D4 kernels (include/d4amd.h). Validates arguments, owns the
+// per-device table allocation and sequences the kernels on the caller's
+// stream. No host synchronisation except in d4_molecule_status.
+
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "d4_internal.h"
+
+struct d4_event_pair {
+  int slot;
+  hipEvent_t start, stop;
+};
+
+struct d4_context {
+  int device;
+  void* mem;
+  d4::DevTables t;
+  // diagnostics: per-kernel event timing
+  bool profile = false;
+  std::vector<d4_event_pair> pending;
+  std::vector<hipEvent_t> pool;
+  double prof_ms[D4_NUM_SLOTS] = {};
+  int64_t prof_count[D4_NUM_SLOTS] = {};
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int set_error(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+int hip_error(hipError_t e, const char* where) {
+  g_error = std::string(where) + ": " + hipGetErrorString(e);
+  return D4_ERR_DEVICE;
+}
+
+#define D4_TRY(expr, where)                      \
+  do {                                           \
+    hipError_t _e = (expr);                      \
+    if (_e != hipSuccess) return hip_error(_e, where); \
+  } while (0)
+
+d4::Params to_params(const d4_params* p) {
+  d4::Params k{};
+  k.s6 = p->s6;
+  k.s8 = p->s8;
+  k.s10 = p->s10;
+  k.s9 = p->s9;
+  k.a1 = p->a1;
+  k.a2 = p->a2;
+  k.alp = p->alp;
+  k.ga = p->ga;
+  k.gc = p->gc;
+  k.wf = p->wf;
+  k.cut2 = p->cutoff_disp2;
+  k.cut3sq = p->cutoff_disp3 * p->cutoff_disp3;
+  k.cutcn = p->cutoff_cn;
+  k.cuteeq = p->cutoff_eeq;
+  k.s10on = (p->flags & D4_FLAG_S10) ? 1 : 0;
+  k.force_lu = (p->flags & D4_FLAG_FORCE_LU) ? 1 : 0;
+  return k;
+}
+
+int check_common(const d4_context* ctx, int64_t nbatch, int64_t natoms,
+                 const int64_t* numbers, const double* positions,
+                 const double* charge, const d4_params* params, void* ws,
+                 size_t ws_bytes) {
+  if (ctx == nullptr) return set_error(D4_ERR_INVALID, "context is NULL");
+  if (params == nullptr) return set_error(D4_ERR_INVALID, "params is NULL");
+  if (nbatch < 0 || natoms < 0)
+    return set_error(D4_ERR_INVALID, "nbatch and natoms must be non-negative");
+  if (natoms > 46000)
+    return set_error(D4_ERR_INVALID, "natoms exceeds the supported maximum (46000)");
+  if (nbatch == 0 || natoms == 0) return D4_OK;
+  if (!numbers || !positions || !charge)
+    return set_error(D4_ERR_INVALID, "numbers, positions and charge must be non-NULL");
+  size_t need = 0;
+  d4_workspace_bytes(nbatch, natoms, &need);
+  if (ws == nullptr || ws_bytes < need)
+    return set_error(D4_ERR_WORKSPACE, "workspace smaller than d4_workspace_bytes()");
+  if (!(params->cutoff_disp2 >= 0.0) || !(params->cutoff_disp3 >= 0.0) ||
+      !(params->cutoff_cn >= 0.0) || !(params->cutoff_eeq >= 0.0))
+    return set_error(D4_ERR_INVALID, "cutoffs must be non-negative");
+  return D4_OK;
+}
+
+struct Call {
+  d4::Work w;
+  d4::Inputs in;
+  d4::Params p;
+  hipStream_t s;
+};
+
+Call make_call(void* stream, int64_t nbatch, int64_t natoms,
+               const int64_t* numbers, const double* positions,
+               const double* charge, const double* q_in, const double* r4r2_in,
+               const d4_params* params, void* ws) {
+  Call c;
+  d4::layout(nbatch, natoms, (char*)ws, &c.w);
+  c.in = d4::Inputs{numbers, positions, charge, q_in, r4r2_in, nullptr};
+  c.p = to_params(params);
+  c.s = (hipStream_t)stream;
+  return c;
+}
+
+hipEvent_t take_event(d4_context* ctx) {
+  if (!ctx->pool.empty()) {
+    hipEvent_t e = ctx->pool.back();
+    ctx->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Launch through `fn`, bracketed by timing events when profiling is on.
+template <class F>
+hipError_t timed(d4_context* ctx, hipStream_t s, int slot, F&& fn) {
+  if (!ctx->profile) return fn();
+  hipEvent_t a = take_event(ctx), b = take_event(ctx);
+  if (!a || !b) return fn();
+  hipError_t e = hipEventRecord(a, s);
+  if (e != hipSuccess) return e;
+  e = fn();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(b, s);
+  if (e != hipSuccess) return e;
+  ctx->pending.push_back({slot, a, b});
+  return hipSuccess;
+}
+
+#define D4_RUN(slot, expr, where) \
+  D4_TRY(timed(ctx, c.s, slot, [&]() { return (expr); }), where)
+
+int run_front(d4_context* ctx, Call& c) {
+  D4_RUN(D4_SLOT_PREP, d4::launch_prep(c.s, c.in, c.w), "prep");
+  D4_RUN(D4_SLOT_CN, d4::launch_cn(c.s, ctx->t, c.p, c.in, c.w), "coordination numbers");
+  D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
+  D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in),
+         "reference weights");
+  return D4_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
+  if (out == nullptr) return set_error(D4_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  if (tb == nullptr) return set_error(D4_ERR_INVALID, "tables is NULL");
+  const double* dsrc[] = {tb->rcov,    tb->en,      tb->r4r2,   tb->eeq_chi,
+                          tb->eeq_eta, tb->eeq_kcn, tb->eeq_rad, tb->gam,
+                          tb->zeff,    tb->maxcn};
+  for (const double* p : dsrc)
+    if (p == nullptr) return set_error(D4_ERR_INVALID, "a per-element table is NULL");
+  if (!tb->nref || !tb->ngw || !tb->refcn || !tb->refq || !tb->alpha0 || !tb->rc6)
+    return set_error(D4_ERR_INVALID, "a reference table is NULL");
+
+  const size_t ne = D4_MAX_Z, nr = size_t(D4_MAX_Z) * D4_MAX_REF;
+  const size_t nrc6 = size_t(D4_MAX_Z) * D4_MAX_Z * D4_MAX_REF * D4_MAX_REF;
+  const size_t ndbl = 10 * ne + 3 * nr + nrc6;
+  const size_t nint = ne + nr;
+  std::vector<double> hd;
+  hd.reserve(ndbl);
+  for (const double* p : dsrc) hd.insert(hd.end(), p, p + ne);
+  hd.insert(hd.end(), tb->refcn, tb->refcn + nr);
+  hd.insert(hd.end(), tb->refq, tb->refq + nr);
+  hd.insert(hd.end(), tb->alpha0, tb->alpha0 + nr);
+  hd.insert(hd.end(), tb->rc6, tb->rc6 + nrc6);
+  std::vector<int32_t> hi;
+  hi.insert(hi.end(), tb->nref, tb->nref + ne);
+  hi.insert(hi.end(), tb->ngw, tb->ngw + nr);
+  for (size_t z = 0; z < ne; ++z)
+    if (hi[z] < 0 || hi[z] > D4_MAX_REF)
+      return set_error(D4_ERR_INVALID, "nref out of range");
+
+  int prev = 0;
+  D4_TRY(hipGetDevice(&prev), "hipGetDevice");
+  D4_TRY(hipSetDevice(device), "hipSetDevice");
+  void* mem = nullptr;
+  const size_t bytes = sizeof(double) * ndbl + sizeof(int32_t) * nint;
+  hipError_t e = hipMalloc(&mem, bytes);
+  if (e != hipSuccess) {
+    (void)hipSetDevice(prev);
+    return set_error(D4_ERR_NOMEM, std::string("hipMalloc tables: ") + hipGetErrorString(e));
+  }
+  e = hipMemcpy(mem, hd.data(), sizeof(double) * ndbl, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy((char*)mem + sizeof(double) * ndbl, hi.data(), sizeof(int32_t) * nint,
+                  hipMemcpyHostToDevice);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    (void)hipFree(mem);
+    return hip_error(e, "upload tables");
+  }
+  d4_context* ctx = new d4_context;
+  ctx->device = device;
+  ctx->mem = mem;
+  const double* d = (const double*)mem;
+  d4::DevTables& t = ctx->t;
+  t.rcov = d + 0 * ne;
+  t.en = d + 1 * ne;
+  t.r4r2 = d + 2 * ne;
+  t.chi = d + 3 * ne;
+  t.eta = d + 4 * ne;
+  t.kcn = d + 5 * ne;
+  t.rad = d + 6 * ne;
+  t.gam = d + 7 * ne;
+  t.zeff = d + 8 * ne;
+  t.maxcn = d + 9 * ne;
+  t.refcn = d + 10 * ne;
+  t.refq = t.refcn + nr;
+  t.alpha0 = t.refq + nr;
+  t.rc6 = t.alpha0 + nr;
+  const int32_t* ip = (const int32_t*)(d + ndbl);
+  t.nref = ip;
+  t.ngw = ip + ne;
+  *out = ctx;
+  return D4_OK;
+}
+
+int d4_context_destroy(d4_context* ctx) {
+  if (ctx == nullptr) return D4_OK;
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(ctx->device);
+  hipError_t e = hipFree(ctx->mem);
+  (void)hipSetDevice(prev);
+  delete ctx;
+  if (e != hipSuccess) return hip_error(e, "hipFree tables");
+  return D4_OK;
+}
+
+int d4_workspace_bytes(int64_t nbatch, int64_t natoms, size_t* bytes) {
+  if (bytes == nullptr) return set_error(D4_ERR_INVALID, "bytes is NULL");
+  if (nbatch < 0 || natoms < 0) return set_error(D4_ERR_INVALID, "negative size");
+  *bytes = d4::layout(nbatch, natoms, nullptr, nullptr);
+  return D4_OK;
+}
+
+int d4_forward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+               const int64_t* numbers, const double* positions, const double* charge,
+               const double* q_in, const double* r4r2_in, const d4_params* params,
+               double* energy_out, double* q_out, void* workspace,
+               size_t workspace_bytes) {
+  int rc = check_common(ctx, nbatch, natoms, numbers, positions, charge, params,
+                        workspace, workspace_bytes);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (energy_out == nullptr) return set_error(D4_ERR_INVALID, "energy_out is NULL");
+  Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
+                     params, workspace);
+  if ((rc = run_front(ctx, c)) != D4_OK) return rc;
+  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, false, nullptr), "pair pass");
+  D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, true, false), "three-body term");
+  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.in, c.w, true, false, energy_out, nullptr),
+         "energy rows");
+  if (q_out) D4_TRY(d4::launch_copy(c.s, c.w.q, q_out, nbatch * natoms), "copy charges");
+  return D4_OK;
+}
+
+int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                const int64_t* numbers, const double* positions, const double* charge,
+                const double* q_in, const double* r4r2_in, const d4_params* params,
+                const double* grad_energy, double* grad_positions, void* workspace,
+                size_t workspace_bytes) {
+  int rc = check_common(ctx, nbatch, natoms, numbers, positions, charge, params,
+                        workspace, workspace_bytes);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (grad_positions == nullptr)
+    return set_error(D4_ERR_INVALID, "grad_positions is NULL");
+  Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
+                     params, workspace);
+  c.in.gE = grad_energy;
+  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, grad_positions),
+         "pair pass (gradient)");
+  D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, false, true), "three-body gradient");
+  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.in, c.w, false, true, nullptr, grad_positions),
+         "three-body rows");
+  D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
+  D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, grad_positions), "CN/EEQ chain");
+  return D4_OK;
+}
+
+int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                       const int64_t* numbers, const double* positions,
+                       const double* charge, const double* q_in, const double* r4r2_in,
+                       const d4_params* params, double* energy_out,
+                       double* grad_positions, void* workspace, size_t workspace_bytes) {
+  int rc = check_common(ctx, nbatch, natoms, numbers, positions, charge, params,
+                        workspace, workspace_bytes);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (energy_out == nullptr || grad_positions == nullptr)
+    return set_error(D4_ERR_INVALID, "energy_out and grad_positions must be non-NULL");
+  Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
+                     params, workspace);
+  if ((rc = run_front(ctx, c)) != D4_OK) return rc;
+  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, grad_positions),
+         "pair pass (gradient)");
+  D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, true, true), "three-body term");
+  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.in, c.w, true, true, energy_out, grad_positions),
+         "three-body rows");
+  D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
+  D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, grad_positions), "CN/EEQ chain");
+  return D4_OK;
+}
+
+int d4_properties(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                  const int64_t* numbers, const double* positions, const double* charge,
+                  const d4_params* params, double* cn_out, double* q_out, double* c6_out,
+                  double* alpha_out, void* workspace, size_t workspace_bytes) {
+  int rc = check_common(ctx, nbatch, natoms, numbers, positions, charge, params,
+                        workspace, workspace_bytes);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, nullptr,
+                     nullptr, params, workspace);
+  if ((rc = run_front(ctx, c)) != D4_OK) return rc;
+  if (c6_out || alpha_out)
+    D4_TRY(d4::launch_properties(c.s, ctx->t, c.in, c.w, c6_out, alpha_out),
+           "properties");
+  if (q_out) D4_TRY(d4::launch_copy(c.s, c.w.q, q_out, nbatch * natoms), "copy charges");
+  if (cn_out)
+    D4_TRY(d4::launch_copy(c.s, c.w.cn, cn_out, nbatch * natoms), "copy coordination numbers");
+  return D4_OK;
+}
+
+int d4_molecule_status(void* stream, int64_t nbatch, const void* workspace,
+                       int32_t* status_out) {
+  if (nbatch < 0 || status_out == nullptr || (nbatch > 0 && workspace == nullptr))
+    return set_error(D4_ERR_INVALID, "bad arguments");
+  if (nbatch == 0) return D4_OK;
+  // status is the first array of the layout
+  D4_TRY(hipMemcpyAsync(status_out, workspace, sizeof(int32_t) * nbatch,
+                        hipMemcpyDeviceToHost, (hipStream_t)stream),
+         "copy status");
+  D4_TRY(hipStreamSynchronize((hipStream_t)stream), "synchronize");
+  return D4_OK;
+}
+
+int d4_profile_enable(d4_context* ctx, int on) {
+  if (ctx == nullptr) return set_error(D4_ERR_INVALID, "context is NULL");
+  ctx->profile = on != 0;
+  return D4_OK;
+}
+
+int d4_profile_read(d4_context* ctx, double* ms_out, int64_t* count_out, int nslots) {
+  if (ctx == nullptr) return set_error(D4_ERR_INVALID, "context is NULL");
+  for (const d4_event_pair& ev : ctx->pending) {
+    D4_TRY(hipEventSynchronize(ev.stop), "hipEventSynchronize");
+    float ms = 0.0f;
+    D4_TRY(hipEventElapsedTime(&ms, ev.start, ev.stop), "hipEventElapsedTime");
+    ctx->prof_ms[ev.slot] += ms;
+    ctx->prof_count[ev.slot] += 1;
+    ctx->pool.push_back(ev.start);
+    ctx->pool.push_back(ev.stop);
+  }
+  ctx->pending.clear();
+  for (int k = 0; k < nslots && k < D4_NUM_SLOTS; ++k) {
+    if (ms_out) ms_out[k] = ctx->prof_ms[k];
+    if (count_out) count_out[k] = ctx->prof_count[k];
+    ctx->prof_ms[k] = 0.0;
+    ctx->prof_count[k] = 0;
+  }
+  return D4_OK;
+}
+
+const char* d4_last_error(void) { return g_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+// Internal declarations shared by the kernel and C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/d4amd.h"
+
+namespace d4 {
+
+constexpr int kMaxZ = D4_MAX_Z;
+constexpr int kMaxRef = D4_MAX_REF;
+constexpr int kRefStride = 8;  // per-atom weight vectors are padded to 8
+constexpr int kRc6Block = kMaxRef * kMaxRef;
+
+// Molecule status bits (d4_molecule_status).
+constexpr int kStatusLU = 1;
+constexpr int kStatusSingular = 2;
+constexpr int kStatusBadZ = 4;
+
+// Device-resident element tables (pointers into one allocation).
+struct DevTables {
+  const double* rcov;
+  const double* en;
+  const double* r4r2;
+  const double* chi;
+  const double* eta;
+  const double* kcn;
+  const double* rad;
+  const double* gam;
+  const double* zeff;
+  const double* maxcn;
+  const double* refcn;
+  const double* refq;
+  const double* alpha0;
+  const double* rc6;
+  const int32_t* nref;
+  const int32_t* ngw;
+};
+
+// Kernel-side copy of the parameters with derived constants.
+struct Params {
+  double s6, s8, s10, s9, a1, a2, alp;
+  double ga, gc, wf;
+  double cut2, cut3sq, cutcn, cuteeq;
+  int s10on;
+  int force_lu;
+};
+
+// Workspace views for one batch (nbatch molecules padded to natoms).
+struct Work {
+  int32_t* status;  // [B]
+  int32_t* natom;   // [B] extent: 1 + index of the last real atom
+  double* cn;       // [B*N] D4 coordination number
+  double* cnq;      // [B*N] capped EEQ coordination number
+  double* dcnq;     // [B*N] d cap / d raw EEQ CN
+  double* q;        // [B*N] charges used
+  double* e2;       // [B*N] two-body atom energies
+  double* dldcn;    // [B*N] dL/dCN
+  double* dldq;     // [B*N] dL/dq
+  double* beta;     // [B*N] lambda_i * d b_i / d raw EEQ CN
+  double* lam;      // [B*N] EEQ adjoint
+  double* gw;       // [B*N*8] weights with charges
+  double* gw0;      // [B*N*8] weights with q = 0 (ATM)
+  double* dgw;      // [B*N*8] d gw / d CN
+  double* dgw0;     // [B*N*8] d gw0 / d CN
+  double* dgwq;     // [B*N*8] d gw / d q
+  double* chol;     // [B*NT] packed lower Cholesky factor of the Coulomb block
+  double* ez;       // [B*N] C^-1 e (e = 1 on real atoms)
+  double* lu;       // [B*(N+1)^2] pivoted LU of the bordered matrix (fallback)
+  int32_t* piv;     // [B*(N+1)]
+  double* vec;      // [B*(N+1)] right-hand side / solution of the LU path
+  double* pt;       // [B*N*N] ATM damping factor t_ij = (R0_ij / r_ij)^(alp/3)
+  double* pp;       // [B*N*N] ATM factor p_ij = sqrt|C6(q=0)_ij| / r_ij^3
+  double* pk;       // [B*N*N] d log sqrt|C6(q=0)_ij| / d CN_i
+  double* pg;       // [B*N*N] ATM dL/d r_ij^2
+  double* ph;       // [B*N*N] ATM dL/d log sqrt|C6(q=0)_ij|
+  double* pe;       // [B*N*N] ATM atom-energy partials (row atom)
+  int64_t B, N;
+};
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Layout of the workspace; returns the total size. With base == nullptr only
+// the size is computed.
+inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> char* {
+    char* p = base ? base + off : nullptr;
+    off = align_up(off + bytes);
+    return p;
+  };
+  const size_t bn = size_t(B) * size_t(N);
+  const size_t nt = size_t(N) * size_t(N + 1) / 2;
+  const size_t n1 = size_t(N + 1);
+  Work v{};
+  v.B = B;
+  v.N = N;
+  v.status = (int32_t*)take(sizeof(int32_t) * B);
+  v.natom = (int32_t*)take(sizeof(int32_t) * B);
+  v.cn = (double*)take(8 * bn);
+  v.cnq = (double*)take(8 * bn);
+  v.dcnq = (double*)take(8 * bn);
+  v.q = (double*)take(8 * bn);
+  v.e2 = (double*)take(8 * bn);
+  v.dldcn = (double*)take(8 * bn);
+  v.dldq = (double*)take(8 * bn);
+  v.beta = (double*)take(8 * bn);
+  v.lam = (double*)take(8 * bn);
+  v.gw = (double*)take(8 * bn * kRefStride);
+  v.gw0 = (double*)take(8 * bn * kRefStride);
+  v.dgw = (double*)take(8 * bn * kRefStride);
+  v.dgw0 = (double*)take(8 * bn * kRefStride);
+  v.dgwq = (double*)take(8 * bn * kRefStride);
+  v.chol = (double*)take(8 * size_t(B) * nt);
+  v.ez = (double*)take(8 * bn);
+  v.lu = (double*)take(8 * size_t(B) * n1 * n1);
+  v.piv = (int32_t*)take(sizeof(int32_t) * size_t(B) * n1);
+  v.vec = (double*)take(8 * size_t(B) * n1);
+  v.pt = (double*)take(8 * bn * size_t(N));
+  v.pp = (double*)take(8 * bn * size_t(N));
+  v.pk = (double*)take(8 * bn * size_t(N));
+  v.pg = (double*)take(8 * bn * size_t(N));
+  v.ph = (double*)take(8 * bn * size_t(N));
+  v.pe = (double*)take(8 * bn * size_t(N));
+  if (w) *w = v;
+  return off;
+}
+
+// Inputs of one call (device pointers).
+struct Inputs {
+  const int64_t* numbers;
+  const double* positions;
+  const double* charge;
+  const double* q_in;     // optional
+  const double* r4r2_in;  // optional
+  const double* gE;       // optional dL/dE_i (nullptr: ones)
+};
+
+// Launchers (d4_kernels.hip). All enqueue on `s` and return hipError_t.
+hipError_t launch_prep(hipStream_t s, const Inputs& in, const Work& w);
+hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
+                     const Inputs& in, const Work& w);
+hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
+                      const Inputs& in, const Work& w, bool keep_factor);
+hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
+                          const Work& w, const Inputs& in);
+hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
+                        const Inputs& in, const Work& w, bool grad,
+                        double* grad_out);
+hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
+                      const Work& w, bool energy, bool grad);
+hipError_t launch_atm_rows(hipStream_t s, const Inputs& in, const Work& w,
+                           bool energy, bool grad, double* energy_out,
+                           double* grad_out);
+hipError_t launch_adjoint(hipStream_t s, const DevTables& t, const Params& p,
+                          const Inputs& in, const Work& w);
+hipError_t launch_chain(hipStream_t s, const DevTables& t, const Params& p,
+                        const Inputs& in, const Work& w, double* grad_out);
+hipError_t launch_properties(hipStream_t s, const DevTables& t,
+                             const Inputs& in, const Work& w, double* c6_out,
+                             double* alpha_out);
+hipError_t launch_copy(hipStream_t s, const double* src, double* dst, int64_t count);
+
+}  // namespace d4

@@ -1,0 +1,236 @@
+"""
+Device runtime: per-device contexts (element tables resident in HBM),
+workspace sizing, and thin wrappers that call the C ABI on torch tensors
+with torch's current HIP stream. No host synchronisation happens here.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .tables import GA_DEFAULT, GC_DEFAULT, WF_DEFAULT, device_tables
+
+_CTX: dict[tuple, ctypes.c_void_p] = {}
+_CTX_LOCK = threading.Lock()
+_KEEPALIVE: list = []
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def context(device: torch.device, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT):
+    """Context for ``device`` and the model constants (ga, gc), created once
+    and kept for the life of the process."""
+    lib = _lib.load()
+    index = device.index if device.index is not None else torch.cuda.current_device()
+    key = (index, float(ga), float(gc))
+    with _CTX_LOCK:
+        if key in _CTX:
+            return _CTX[key]
+        t = device_tables(float(ga), float(gc))
+        arrays = [t.rcov, t.en, t.r4r2, t.eeq_chi, t.eeq_eta, t.eeq_kcn, t.eeq_rad,
+                  t.gam, t.zeff, t.maxcn, t.refcn, t.refq, t.alpha0, t.rc6, t.nref, t.ngw]
+        _KEEPALIVE.append(arrays)
+        tb = _lib.Tables(
+            rcov=_ptr(t.rcov, ctypes.c_double),
+            en=_ptr(t.en, ctypes.c_double),
+            r4r2=_ptr(t.r4r2, ctypes.c_double),
+            eeq_chi=_ptr(t.eeq_chi, ctypes.c_double),
+            eeq_eta=_ptr(t.eeq_eta, ctypes.c_double),
+            eeq_kcn=_ptr(t.eeq_kcn, ctypes.c_double),
+            eeq_rad=_ptr(t.eeq_rad, ctypes.c_double),
+            gam=_ptr(t.gam, ctypes.c_double),
+            zeff=_ptr(t.zeff, ctypes.c_double),
+            maxcn=_ptr(t.maxcn, ctypes.c_double),
+            nref=_ptr(t.nref, ctypes.c_int32),
+            ngw=_ptr(t.ngw, ctypes.c_int32),
+            refcn=_ptr(t.refcn, ctypes.c_double),
+            refq=_ptr(t.refq, ctypes.c_double),
+            alpha0=_ptr(t.alpha0, ctypes.c_double),
+            rc6=_ptr(t.rc6, ctypes.c_double),
+        )
+        out = ctypes.c_void_p()
+        _lib.check(lib.d4_context_create(index, ctypes.byref(tb), ctypes.byref(out)),
+                   "d4_context_create")
+        _CTX[key] = out
+        return out
+
+
+def workspace_bytes(nbatch: int, natoms: int) -> int:
+    n = ctypes.c_size_t()
+    _lib.check(_lib.load().d4_workspace_bytes(nbatch, natoms, ctypes.byref(n)),
+               "d4_workspace_bytes")
+    return int(n.value)
+
+
+def new_workspace(nbatch: int, natoms: int, device) -> torch.Tensor:
+    return torch.empty(workspace_bytes(nbatch, natoms), dtype=torch.uint8, device=device)
+
+
+@dataclass
+class KernelParams:
+    """Damping parameters, model constants and cutoffs for one call."""
+
+    s6: float = 1.0
+    s8: float = 0.0
+    s10: float = 0.0
+    s9: float = 1.0
+    a1: float = 0.4
+    a2: float = 5.0
+    alp: float = 16.0
+    ga: float = GA_DEFAULT
+    gc: float = GC_DEFAULT
+    wf: float = WF_DEFAULT
+    cutoff_disp2: float = 60.0
+    cutoff_disp3: float = 40.0
+    cutoff_cn: float = 30.0
+    cutoff_eeq: float = 25.0
+    s10_on: bool = False
+    force_lu: bool = False
+
+    def c(self) -> _lib.Params:
+        flags = (_lib.D4_FLAG_S10 if self.s10_on else 0) | (
+            _lib.D4_FLAG_FORCE_LU if self.force_lu else 0
+        )
+        return _lib.Params(
+            self.s6, self.s8, self.s10, self.s9, self.a1, self.a2, self.alp,
+            self.ga, self.gc, self.wf, self.cutoff_disp2, self.cutoff_disp3,
+            self.cutoff_cn, self.cutoff_eeq, flags, 0,
+        )
+
+
+def _vp(t: torch.Tensor | None):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _check_inputs(numbers, positions, charge):
+    if not positions.is_cuda:
+        raise RuntimeError("tad_dftd4_amd runs on a ROCm device; got CPU tensors")
+    assert numbers.dtype == torch.int64 and numbers.is_contiguous()
+    assert positions.dtype == torch.float64 and positions.is_contiguous()
+    assert charge.dtype == torch.float64 and charge.is_contiguous()
+    assert numbers.device == positions.device == charge.device
+
+
+def forward(numbers, positions, charge, kp: KernelParams, q=None, r4r2=None,
+            ws=None, q_out=None):
+    """Atom energies (B, N); returns (energy, workspace)."""
+    _check_inputs(numbers, positions, charge)
+    B, N = numbers.shape
+    dev = positions.device
+    ctx = context(dev, kp.ga, kp.gc)
+    if ws is None:
+        ws = new_workspace(B, N, dev)
+    energy = torch.empty((B, N), dtype=torch.float64, device=dev)
+    p = kp.c()
+    _lib.check(
+        _lib.load().d4_forward(
+            ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
+            _vp(r4r2), ctypes.byref(p), _vp(energy), _vp(q_out), _vp(ws), ws.numel()),
+        "d4_forward",
+    )
+    return energy, ws
+
+
+def backward(numbers, positions, charge, kp: KernelParams, ws, grad_energy=None,
+             q=None, r4r2=None):
+    """Gradient of sum(grad_energy * E) w.r.t. positions, (B, N, 3)."""
+    _check_inputs(numbers, positions, charge)
+    B, N = numbers.shape
+    dev = positions.device
+    ctx = context(dev, kp.ga, kp.gc)
+    grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
+    if grad_energy is not None:
+        grad_energy = grad_energy.to(torch.float64).contiguous()
+    p = kp.c()
+    _lib.check(
+        _lib.load().d4_backward(
+            ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
+            _vp(r4r2), ctypes.byref(p), _vp(grad_energy), _vp(grad), _vp(ws), ws.numel()),
+        "d4_backward",
+    )
+    return grad
+
+
+def energy_gradient(numbers, positions, charge, kp: KernelParams, q=None, r4r2=None,
+                    ws=None, energy=None, grad=None):
+    """Fused atom energies and gradient of the total energy."""
+    _check_inputs(numbers, positions, charge)
+    B, N = numbers.shape
+    dev = positions.device
+    ctx = context(dev, kp.ga, kp.gc)
+    if ws is None:
+        ws = new_workspace(B, N, dev)
+    if energy is None:
+        energy = torch.empty((B, N), dtype=torch.float64, device=dev)
+    if grad is None:
+        grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
+    p = kp.c()
+    _lib.check(
+        _lib.load().d4_energy_gradient(
+            ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
+            _vp(r4r2), ctypes.byref(p), _vp(energy), _vp(grad), _vp(ws), ws.numel()),
+        "d4_energy_gradient",
+    )
+    return energy, grad, ws
+
+
+def properties(numbers, positions, charge, kp: KernelParams, ws=None):
+    """(cn, q, c6, alpha) as in get_properties."""
+    _check_inputs(numbers, positions, charge)
+    B, N = numbers.shape
+    dev = positions.device
+    ctx = context(dev, kp.ga, kp.gc)
+    if ws is None:
+        ws = new_workspace(B, N, dev)
+    f64 = dict(dtype=torch.float64, device=dev)
+    cn = torch.empty((B, N), **f64)
+    q = torch.empty((B, N), **f64)
+    c6 = torch.empty((B, N, N), **f64)
+    alpha = torch.empty((B, N), **f64)
+    p = kp.c()
+    _lib.check(
+        _lib.load().d4_properties(
+            ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge),
+            ctypes.byref(p), _vp(cn), _vp(q), _vp(c6), _vp(alpha), _vp(ws), ws.numel()),
+        "d4_properties",
+    )
+    return cn, q, c6, alpha
+
+
+def molecule_status(ws: torch.Tensor, nbatch: int) -> np.ndarray:
+    """Per-molecule status bits of the last call on ``ws`` (synchronises)."""
+    out = np.zeros(nbatch, dtype=np.int32)
+    _lib.check(
+        _lib.load().d4_molecule_status(_stream(ws.device), nbatch, _vp(ws),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+        "d4_molecule_status",
+    )
+    return out
+
+
+def profile_enable(device, on: bool, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT) -> None:
+    """Per-kernel HIP-event timing of calls through the context of ``device``."""
+    _lib.check(_lib.load().d4_profile_enable(context(device, ga, gc), int(bool(on))),
+               "d4_profile_enable")
+
+
+def profile_read(device, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT) -> dict:
+    """{kernel: (total_ms, launches)} since the last read (synchronises)."""
+    n = len(_lib.SLOTS)
+    ms = (ctypes.c_double * n)()
+    cnt = (ctypes.c_int64 * n)()
+    _lib.check(_lib.load().d4_profile_read(context(device, ga, gc), ms, cnt, n),
+               "d4_profile_read")
+    return {name: (ms[k], cnt[k]) for k, name in enumerate(_lib.SLOTS)}

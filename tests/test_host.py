@@ -1,0 +1,127 @@
+"""
+CPU tests of the host side: the C-ABI library loads and exports every
+symbol declared in include/d4amd.h (no compute calls), the device tables
+built on the host match the oracle, and the Python boundary follows the
+reference's argument conventions and error types.
+"""
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import tad_dftd4_amd as d4
+from oracle import dense
+from tad_dftd4_amd import _lib, tables
+from tad_dftd4_amd.disp import kernel_params
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "d4amd.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(d4_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_header_symbols():
+    lib = _lib.load()
+    names = _declared_symbols()
+    assert len(names) >= 9
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_lib.SIGNATURES)
+
+
+def test_workspace_bytes_and_errors():
+    lib = _lib.load()
+    n = ctypes.c_size_t()
+    assert lib.d4_workspace_bytes(4, 10, ctypes.byref(n)) == 0
+    small = n.value
+    assert lib.d4_workspace_bytes(4, 20, ctypes.byref(n)) == 0
+    assert n.value > small
+    assert lib.d4_workspace_bytes(-1, 3, ctypes.byref(n)) == _lib.D4_ERR_INVALID
+    assert b"negative" in lib.d4_last_error()
+    # argument validation happens before any device work
+    p = _lib.Params()
+    rc = lib.d4_forward(None, None, 1, 1, None, None, None, None, None, ctypes.byref(p),
+                        None, None, None, 0)
+    assert rc == _lib.D4_ERR_INVALID and b"context" in lib.d4_last_error()
+
+
+def test_context_rejects_null_tables():
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    tb = _lib.Tables()
+    assert lib.d4_context_create(0, ctypes.byref(tb), ctypes.byref(out)) == _lib.D4_ERR_INVALID
+
+
+def test_device_tables_match_oracle_rc6():
+    """Compacted element-pair C6 table equals the oracle's per-atom rc6."""
+    t = tables.device_tables()
+    numbers = torch.tensor([1, 6, 7, 8, 16, 3, 14, 26, 53, 92])
+    rc6 = dense.get_refc6(numbers).numpy()
+    for a, za in enumerate(numbers.tolist()):
+        for b, zb in enumerate(numbers.tolist()):
+            pa = t.perm[za, : t.nref[za]]
+            pb = t.perm[zb, : t.nref[zb]]
+            want = rc6[a, b][np.ix_(pa, pb)]
+            got = t.rc6[za, zb, : len(pa), : len(pb)]
+            assert np.allclose(got, want, rtol=1e-13, atol=0)
+            assert np.all(t.rc6[za, zb, len(pa):, :] == 0)
+
+
+def test_device_tables_reference_data():
+    t = tables.device_tables()
+    assert t.nref[1] == 2 and t.nref[6] == 7 and t.nref[92] == 6
+    assert np.all(t.ngw[1, :2] == [3, 1])
+    alpha = dense.get_alpha(torch.tensor([6])).numpy()[0]
+    assert np.allclose(t.alpha0[6, :7], alpha[t.perm[6, :7], 0], rtol=1e-14)
+    assert np.allclose(t.r4r2, dense.tables().r4r2.numpy()[:104], rtol=1e-15)
+
+
+def test_kernel_params_conventions():
+    kp = kernel_params({"a1": 0.4, "a2": 5.0}, None, None)
+    assert kp.s8 == 0.0 and not kp.s10_on and kp.s6 == 1.0 and kp.s9 == 1.0
+    assert kp.alp == 16.0 and kp.cutoff_disp2 == 60.0 and kp.cutoff_disp3 == 40.0
+    kp = kernel_params({"a1": 0.4, "a2": 5.0, "s8": torch.tensor(1.2, dtype=torch.float64), "s10": 0.0},
+                       d4.Cutoff(disp2=50, disp3=25, cn=3, cn_eeq=20), None)
+    assert kp.s8 == 1.2 and kp.s10_on and kp.cutoff_eeq == 20.0
+    assert kp.cutoff_cn == 30.0  # Cutoff.cn is ignored, as in the reference
+    with pytest.raises(TypeError):
+        kernel_params({"a1": 0.4}, None, None)
+
+
+def test_boundary_errors_before_device_work():
+    numbers = torch.tensor([6, 1])
+    positions = torch.zeros(2, 3, dtype=torch.float64)
+    par = {"a1": 0.4, "a2": 5.0}
+    with pytest.raises(ValueError):
+        d4.dftd4(numbers, torch.zeros(3, 3), 0.0, par)
+    with pytest.raises(ValueError):
+        d4.dftd4(numbers, positions, 0.0, par, model="d9")
+    with pytest.raises(NotImplementedError):
+        d4.dftd4(numbers, positions, 0.0, par, model="d4s")
+    with pytest.raises(ValueError):
+        d4.dftd4(numbers, positions, 0.0, par, r4r2=torch.ones(3))
+    with pytest.raises(ValueError):
+        d4.dftd4(numbers, positions, 0.0, par, q=torch.ones(3))
+    with pytest.raises(NotImplementedError):
+        d4.dftd4(numbers, positions, 0.0, par, damping_function=d4.ZeroDamping())
+    # CPU tensors are rejected: there is no CPU fallback
+    with pytest.raises(RuntimeError):
+        d4.dftd4(numbers, positions, 0.0, par)
+
+
+def test_cutoff_and_model_objects():
+    c = d4.Cutoff()
+    assert float(c.disp2) == 60.0 and float(c.cn_eeq) == 25.0
+    m = d4.D4Model(torch.tensor([6, 1]))
+    assert m.wf == 6.0 and m.ga == 3.0 and m.gc == 2.0
+    rc6 = m.rc6
+    ref = dense.get_refc6(torch.tensor([6, 1])).to(rc6.dtype)
+    assert torch.allclose(rc6.double(), ref.double(), rtol=1e-6)
+    with pytest.raises(NotImplementedError):
+        d4.D4Model(torch.tensor([6]), ref_charges="gfn2")

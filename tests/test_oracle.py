@@ -302,3 +302,31 @@ def test_finite_difference_gradient():
         fd = (dense.dftd4(numbers, pp, m.charge, par).sum()
               - dense.dftd4(numbers, pm, m.charge, par).sum()) / (2 * h)
         assert abs(fd.item() - g[i, x].item()) < 1e-10
+
+
+def test_exceptional_weights_reference_gradient_is_nan():
+    """When every Gaussian weight of an atom underflows (its CN is far from
+    all reference CNs), the reference falls back to the maximum-CN reference
+    (model/d4.py:209-217). Its autograd gradient is then NaN (0/0 in the
+    discarded branch of torch.where). The NaN-safe oracle mode keeps the
+    energies bit-identical and gives the finite gradient, which matches
+    central finite differences; the GPU backward is checked against it."""
+    mol = synth.molecule(np.random.default_rng(3), 64, synth.MIX_Z54, 2.8, (-1, 0, 1))
+    assert 41 in mol.numbers.tolist()  # Nb, CN ~12 away from its references
+    numbers, positions = torch.tensor(mol.numbers), torch.tensor(mol.positions)
+    par = _p({"s6": 1.0, "s8": 1.2, "s9": 1.0, "a1": 0.4, "a2": 5.0})
+    cut = {"disp2": 50.0, "disp3": 25.0, "cn_eeq": 25.0}
+    e_safe, g_safe = dense.dftd4_with_grad(numbers, positions, mol.charge, par, cut)
+    e_ref, g_ref = dense.dftd4_with_grad(numbers, positions, mol.charge, par, cut,
+                                         nan_safe=False)
+    assert torch.equal(e_safe, e_ref)
+    assert torch.isnan(g_ref).any() and torch.isfinite(g_safe).all()
+    i = mol.numbers.tolist().index(41)
+    h = 1e-5
+    for x in range(3):
+        pp, pm = positions.clone(), positions.clone()
+        pp[i, x] += h
+        pm[i, x] -= h
+        fd = (dense.dftd4(numbers, pp, mol.charge, par, cutoff=cut).sum()
+              - dense.dftd4(numbers, pm, mol.charge, par, cutoff=cut).sum()) / (2 * h)
+        assert abs(fd.item() - g_safe[i, x].item()) < 1e-9
