@@ -305,6 +305,44 @@ def get_refc6(numbers: Tensor, dtype=torch.float64) -> Tensor:
     return trapzd(get_alpha(numbers, dtype))
 
 
+class _StableRatio(torch.autograd.Function):
+    """expw / norm of ``weight_references`` with the reference's forward
+    (bit-identical values) and a backward that stays finite where the norm is
+    denormal or zero: d(expw_a/norm)/dcn = (dexpw_a - w_a dnorm) / norm, the
+    ratio w_a being formed before any product with dnorm. Exceptional slots
+    get zero derivative (their value is the constant fallback)."""
+
+    @staticmethod
+    def forward(ctx, cn, refcn, refc, mask, wf, exceptional):
+        dcn = cn.unsqueeze(-1).type(torch.double) - refcn
+        tmp = torch.exp(-dcn * dcn)
+        zero = torch.tensor(0.0, dtype=torch.double)
+
+        def series(n, deriv):
+            out = torch.zeros_like(tmp)
+            for i in range(1, n + 1):
+                term = torch.pow(tmp, i * wf)
+                out = out + (-2.0 * i * wf * dcn * term if deriv else term)
+            return out
+
+        e1, e3 = series(1, False), series(3, False)
+        expw = torch.where(mask, torch.where(refc == 3, e3, torch.where(refc == 1, e1, tmp)), zero)
+        d1, d3 = series(1, True), series(3, True)
+        dexpw = torch.where(mask, torch.where(refc == 3, d3, torch.where(refc == 1, d1, zero)), zero)
+        norm = torch.where(mask, torch.sum(expw, dim=-1, keepdim=True),
+                           torch.tensor(1e-300, dtype=torch.double))
+        w = expw / norm
+        dnorm = torch.sum(dexpw, dim=-1, keepdim=True)
+        dw = torch.where(exceptional | ~mask, zero, (dexpw - torch.where(exceptional, zero, w) * dnorm) / norm)
+        ctx.save_for_backward(dw)
+        return w
+
+    @staticmethod
+    def backward(ctx, grad):
+        (dw,) = ctx.saved_tensors
+        return (grad * dw).sum(-1).type(grad.dtype), None, None, None, None, None
+
+
 def weight_references(numbers: Tensor, cn: Tensor | None = None,
                       q: Tensor | None = None, wf: float = WF,
                       dtype=torch.float64, nan_safe: bool = False) -> Tensor:
@@ -348,8 +386,7 @@ def weight_references(numbers: Tensor, cn: Tensor | None = None,
     maxcn = torch.max(refcn, dim=-1, keepdim=True)[0]
     exceptional = torch.isnan(gw_temp) | (gw_temp > torch.finfo(dtype).max)
     if nan_safe:
-        safe_norm = torch.where(exceptional, torch.ones_like(norm), norm)
-        gw_temp = (expw / safe_norm).type(dtype)
+        gw_temp = _StableRatio.apply(cn, refcn, refc, mask, wf, exceptional).type(dtype)
     gw = torch.where(
         exceptional,
         torch.where(refcn == maxcn, torch.tensor(1.0, dtype=dtype), zero),

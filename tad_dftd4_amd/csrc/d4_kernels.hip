@@ -545,8 +545,10 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
         g = t.refcn[z * kMaxRef + a] == maxcn ? 1.0 : 0.0;
         dg = 0.0;
       } else {
+        // form the ratio first: expw * dnorm underflows when the Gaussians
+        // are far below 1 (CN several units from every reference CN)
         g = expw[a] / norm;
-        dg = (dexpw[a] - expw[a] * dnorm / norm) / norm;
+        dg = (dexpw[a] - g * dnorm) / norm;
       }
       const double qref = t.refq[z * kMaxRef + a] + zeff;
       const double zq = zeta_fn(p.ga, gam, qref, q + zeff);
