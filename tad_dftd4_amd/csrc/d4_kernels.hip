@@ -195,10 +195,210 @@ __device__ __forceinline__ double eeq_diag(const DevTables& t, int z) {
   return t.eta[z] + sqrt(2.0 / kPi) / t.rad[z];
 }
 
-__global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p,
-                                                  Inputs in, Work w,
-                                                  int use_lds) {
+// ---- blocked Cholesky (16-column blocks) on a packed lower triangle -------
+//
+// Row i of the factor starts at L + tri(i). The diagonal 16 x 16 block is
+// factorised by wave 0 in registers (lane r holds row r; columns travel by
+// cross-lane shuffles, so no barrier is needed inside the block); the panel
+// rows below it are then solved independently by all threads, and the
+// trailing triangle is updated in 2 x 2 register tiles. Three barriers per
+// block instead of three per column.
+
+constexpr int kNB = 16;
+
+// Factor the diagonal block at kb (nb <= 16 rows) with wave 0; writes the
+// block back, stores 1/L_ii in dinv, and returns 1 (on wave 0) if a pivot was
+// not positive.
+__device__ __forceinline__ int chol_diag_block(double* L, int kb, int nb, double* dinv) {
+  const int lane = threadIdx.x & 63;
+  double d[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c)
+    d[c] = (lane < nb && c <= lane) ? L[tri(kb + lane) + kb + c] : 0.0;
+  int bad = 0;
+#pragma unroll
+  for (int j = 0; j < kNB; ++j) {
+    if (j < nb) {
+      const double piv = __shfl(d[j], j, 64);
+      bad |= !(piv > 0.0);
+      const double sj = sqrt(fmax(piv, 0.0));
+      const double inv = piv > 0.0 ? 1.0 / sj : 0.0;
+      if (lane == j)
+        d[j] = sj;
+      else if (lane > j)
+        d[j] *= inv;
+#pragma unroll
+      for (int c = j + 1; c < kNB; ++c) {
+        if (c < nb) {
+          const double lcj = __shfl(d[j], c, 64);
+          if (lane >= c) d[c] -= d[j] * lcj;
+        }
+      }
+    }
+  }
+  if (lane < nb) {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (c <= lane) L[tri(kb + lane) + kb + c] = d[c];
+    dinv[kb + lane] = 1.0 / d[lane];
+  }
+  return bad;
+}
+
+// In-place row-vector solve v <- v D^-T for the nb x nb lower block D at kb;
+// the result is also copied to out (if non-null). Solved entries are read
+// back from v, which keeps register pressure low.
+__device__ __forceinline__ void block_fsolve(const double* L, int kb, int nb, const double* dinv,
+                                             double* v, double* out) {
+  for (int c = 0; c < nb; ++c) {
+    const double* Lc = L + tri(kb + c) + kb;
+    double sacc = v[c];
+    for (int d = 0; d < c; ++d) sacc -= v[d] * Lc[d];
+    const double x = sacc * dinv[kb + c];
+    v[c] = x;
+    if (out) out[c] = x;
+  }
+}
+
+// Backward solve of one block in place: v_c <- (v_c - sum_{d > c}
+// L[kb+d][kb+c] v_d) / L_cc, copied to out.
+__device__ __forceinline__ void block_bsolve(const double* L, int kb, int nb, const double* dinv,
+                                             double* v, double* out) {
+  for (int c = nb - 1; c >= 0; --c) {
+    double sacc = v[kb + c];
+    for (int d = c + 1; d < nb; ++d) sacc -= L[tri(kb + d) + kb + c] * v[kb + d];
+    const double x = sacc * dinv[kb + c];
+    v[kb + c] = x;
+    out[c] = x;
+  }
+}
+
+// Backward substitution L^T x = y for up to two right-hand sides (v1 may be
+// null), blocked; overwrites v0/v1. Needs 2 * kNB doubles of LDS in yblk.
+__device__ void chol_backward(const double* L, int n, const double* dinv, double* v0,
+                              double* v1, double* yblk) {
+  const int tid = threadIdx.x;
+  for (int kb = ((n - 1) / kNB) * kNB; kb >= 0; kb -= kNB) {
+    const int nb = min(kNB, n - kb);
+    __syncthreads();
+    if (tid == 0) block_bsolve(L, kb, nb, dinv, v0, yblk);
+    if (tid == 1 && v1 != nullptr) block_bsolve(L, kb, nb, dinv, v1, yblk + kNB);
+    __syncthreads();
+    for (int i = tid; i < kb; i += 256) {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) {
+        if (c < nb) {
+          const double l = L[tri(kb + c) + i];
+          s0 += l * yblk[c];
+          s1 += l * yblk[kNB + c];
+        }
+      }
+      v0[i] -= s0;
+      if (v1 != nullptr) v1[i] -= s1;
+    }
+  }
+  __syncthreads();
+}
+
+// Forward substitution L y = v (one right-hand side), blocked.
+__device__ void chol_forward(const double* L, int n, const double* dinv, double* v,
+                             double* yblk) {
+  const int tid = threadIdx.x;
+  for (int kb = 0; kb < n; kb += kNB) {
+    const int nb = min(kNB, n - kb);
+    __syncthreads();
+    if (tid == 0) block_fsolve(L, kb, nb, dinv, v + kb, yblk);
+    __syncthreads();
+    for (int i = kb + nb + tid; i < n; i += 256) {
+      const double* Li = L + tri(i) + kb;
+      double sacc = 0.0;
+#pragma unroll
+      for (int c = 0; c < kNB; ++c)
+        if (c < nb) sacc += Li[c] * yblk[c];
+      v[i] -= sacc;
+    }
+  }
+  __syncthreads();
+}
+
+// Blocked right-looking Cholesky of the assembled n x n lower triangle with
+// two right-hand sides carried along (forward substitution fused): on return
+// rb, re hold L^-1 b and L^-1 e. Returns non-zero if a pivot was not positive.
+__device__ int chol_factor(double* L, int n, double* dinv, double* rb, double* re,
+                           double* yblk, double* flag) {
+  const int tid = threadIdx.x;
+  for (int kb = 0; kb < n; kb += kNB) {
+    const int nb = min(kNB, n - kb);
+    if (tid < 64) {
+      const int bad = chol_diag_block(L, kb, nb, dinv);
+      if (tid == 0) flag[0] = bad ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (flag[0] != 0.0) return 1;  // uniform
+    const int mr = n - kb - nb;
+    // panel rows and the two right-hand-side blocks
+    for (int t2 = tid; t2 < mr + 2; t2 += 256) {
+      if (t2 < mr)
+        block_fsolve(L, kb, nb, dinv, L + tri(kb + nb + t2) + kb, nullptr);
+      else
+        block_fsolve(L, kb, nb, dinv, (t2 == mr ? rb : re) + kb, yblk + (t2 - mr) * kNB);
+    }
+    __syncthreads();
+    // trailing update in 2 x 2 tiles, plus the right-hand sides
+    const int mt = (mr + 1) / 2;
+    const int ntl = mt * (mt + 1) / 2;
+    for (int e = tid; e < ntl + mr; e += 256) {
+      if (e < ntl) {
+        int ti = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+        while (ti * (ti + 1) / 2 > e) --ti;
+        while ((ti + 1) * (ti + 2) / 2 <= e) ++ti;
+        const int tj = e - ti * (ti + 1) / 2;
+        const int i0 = kb + nb + 2 * ti, j0 = kb + nb + 2 * tj;
+        const bool i1 = i0 + 1 < n, j1 = j0 + 1 < n;
+        const double* Pi0 = L + tri(i0) + kb;
+        const double* Pi1 = i1 ? L + tri(i0 + 1) + kb : Pi0;
+        const double* Pj0 = L + tri(j0) + kb;
+        const double* Pj1 = j1 ? L + tri(j0 + 1) + kb : Pj0;
+        double a00 = 0.0, a10 = 0.0, a01 = 0.0, a11 = 0.0;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+          if (c < nb) {
+            const double x0 = Pi0[c], x1 = Pi1[c], y0 = Pj0[c], y1 = Pj1[c];
+            a00 += x0 * y0;
+            a10 += x1 * y0;
+            a01 += x0 * y1;
+            a11 += x1 * y1;
+          }
+        }
+        L[tri(i0) + j0] -= a00;
+        if (i1) L[tri(i0 + 1) + j0] -= a10;
+        if (j1 && j0 + 1 <= i0) L[tri(i0) + j0 + 1] -= a01;
+        if (i1 && j1) L[tri(i0 + 1) + j0 + 1] -= a11;
+      } else {
+        const int i = kb + nb + (e - ntl);
+        const double* Pi = L + tri(i) + kb;
+        double sb = 0.0, se = 0.0;
+#pragma unroll
+        for (int c = 0; c < kNB; ++c)
+          if (c < nb) {
+            sb += Pi[c] * yblk[c];
+            se += Pi[c] * yblk[kNB + c];
+          }
+        rb[i] -= sb;
+        re[i] -= se;
+      }
+    }
+    __syncthreads();
+  }
+  return 0;
+}
+
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p, Inputs in,
+                                                  Work w) {
   extern __shared__ double smem[];
+  __shared__ double s_small[2 * kNB + 2 * kWaves + 2];
   const int64_t m = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t N = w.N;
@@ -215,21 +415,13 @@ __global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p,
   for (int i = n + tid; i < N; i += 256) q[i] = 0.0;
   if (n == 0) return;
 
-  double* L;
-  double *rb, *re, *colk, *red;
-  if (use_lds) {
-    L = smem;
-    rb = smem + (size_t)tri((int)N);
-    re = rb + N;
-    colk = re + N;
-    red = colk + N;
-  } else {
-    L = w.chol + (size_t)m * (size_t)tri((int)N);
-    rb = w.lam + m * N;  // scratch, overwritten by the backward
-    re = w.ez + m * N;
-    colk = w.beta + m * N;  // scratch
-    red = smem;
-  }
+  double* L = LDS ? smem : w.chol + (size_t)m * (size_t)tri((int)N);
+  double* rb = LDS ? smem + (size_t)tri((int)N) : w.lam + m * N;  // scratch until the adjoint
+  double* re = LDS ? rb + N : w.ez + m * N;
+  double* dinv = LDS ? re + N : w.beta + m * N;  // scratch until the adjoint
+  double* yblk = s_small;
+  double* red = s_small + 2 * kNB;
+  double* flag = red + 2 * kWaves;
 
   // assemble the lower triangle of C and the two right-hand sides
   for (int i = wave; i < n; i += kWaves) {
@@ -256,71 +448,12 @@ __global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p,
   }
   __syncthreads();
 
-  int fail = p.force_lu;
-  if (!fail) {
-    for (int k = 0; k < n; ++k) {
-      const double d = L[tri(k) + k];
-      if (!(d > 0.0)) {  // uniform: every thread reads the same value
-        fail = 1;
-        break;
-      }
-      const double sd = sqrt(d), inv = 1.0 / sd;
-      __syncthreads();
-      for (int i = k + 1 + tid; i < n; i += 256) {
-        const double v = L[tri(i) + k] * inv;
-        L[tri(i) + k] = v;
-        colk[i] = v;
-      }
-      if (tid == 0) L[tri(k) + k] = sd;
-      __syncthreads();
-      for (int i = k + 1 + wave; i < n; i += kWaves) {
-        const double lik = colk[i];
-        double* Li = L + tri(i);
-        for (int j = k + 1 + lane; j <= i; j += 64) Li[j] -= lik * colk[j];
-      }
-      __syncthreads();
-    }
-  }
-  if (fail) {
+  if (p.force_lu || chol_factor(L, n, dinv, rb, re, yblk, flag)) {
     if (tid == 0) w.status[m] |= kStatusLU;
     return;
   }
+  chol_backward(L, n, dinv, rb, re, yblk);
 
-  // forward substitution L y = [b, e] (in place)
-  for (int k = 0; k < n; ++k) {
-    __syncthreads();
-    const double dk = L[tri(k) + k];
-    const double yb = rb[k] / dk, ye = re[k] / dk;
-    for (int i = k + 1 + tid; i < n; i += 256) {
-      const double l = L[tri(i) + k];
-      rb[i] -= l * yb;
-      re[i] -= l * ye;
-    }
-  }
-  __syncthreads();
-  for (int i = tid; i < n; i += 256) {
-    const double d = L[tri(i) + i];
-    rb[i] /= d;
-    re[i] /= d;
-  }
-  // backward substitution L^T x = y (column oriented, in place)
-  for (int k = n - 1; k >= 0; --k) {
-    __syncthreads();
-    const double dk = L[tri(k) + k];
-    const double xb = rb[k] / dk, xe = re[k] / dk;
-    const double* Lk = L + tri(k);
-    for (int i = tid; i < k; i += 256) {
-      rb[i] -= Lk[i] * xb;
-      re[i] -= Lk[i] * xe;
-    }
-  }
-  __syncthreads();
-  for (int i = tid; i < n; i += 256) {
-    const double d = L[tri(i) + i];
-    rb[i] /= d;
-    re[i] /= d;
-  }
-  __syncthreads();
   double sb = 0.0, se = 0.0;
   for (int i = tid; i < n; i += 256) {
     if (Z[i] != 0) {
@@ -332,7 +465,7 @@ __global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p,
   se = block_sum(se, red + kWaves);
   const double mu = (sb - in.charge[m]) / se;
   for (int i = tid; i < n; i += 256) q[i] = Z[i] != 0 ? rb[i] - mu * re[i] : 0.0;
-  if (use_lds) {
+  if (LDS) {
     // keep z = C^-1 e and the factor for the adjoint solve
     double* gz = w.ez + m * N;
     for (int i = tid; i < n; i += 256) gz[i] = re[i];
@@ -934,6 +1067,7 @@ __global__ __launch_bounds__(256) void k_atm_rows(Inputs in, Work w,
 __global__ __launch_bounds__(256) void k_adjoint(DevTables t, Params p, Inputs in,
                                                  Work w) {
   __shared__ double red[2 * kWaves];
+  __shared__ double yblk[2 * kNB];
   const int64_t m = blockIdx.x;
   const int64_t N = w.N;
   const int n = w.natom[m];
@@ -965,24 +1099,14 @@ __global__ __launch_bounds__(256) void k_adjoint(DevTables t, Params p, Inputs i
   } else {
     const double* L = w.chol + (size_t)m * (size_t)tri((int)N);
     const double* ez = w.ez + m * N;
-    // w = C^-1 g in lam (in place, as in k_eeq_chol)
-    for (int i = tid; i < n; i += 256) lam[i] = Z[i] != 0 ? g[i] : 0.0;
-    for (int k = 0; k < n; ++k) {
-      __syncthreads();
-      const double yk = lam[k] / L[tri(k) + k];
-      for (int i = k + 1 + tid; i < n; i += 256) lam[i] -= L[tri(i) + k] * yk;
+    double* dinv = w.beta + m * N;  // scratch: 1/L_ii, beta is written last
+    for (int i = tid; i < n; i += 256) {
+      lam[i] = Z[i] != 0 ? g[i] : 0.0;
+      dinv[i] = 1.0 / L[tri(i) + i];
     }
-    __syncthreads();
-    for (int i = tid; i < n; i += 256) lam[i] /= L[tri(i) + i];
-    for (int k = n - 1; k >= 0; --k) {
-      __syncthreads();
-      const double xk = lam[k] / L[tri(k) + k];
-      const double* Lk = L + tri(k);
-      for (int i = tid; i < k; i += 256) lam[i] -= Lk[i] * xk;
-    }
-    __syncthreads();
-    for (int i = tid; i < n; i += 256) lam[i] /= L[tri(i) + i];
-    __syncthreads();
+    // w = C^-1 g in lam
+    chol_forward(L, n, dinv, lam, yblk);
+    chol_backward(L, n, dinv, lam, nullptr, yblk);
     double sw = 0.0, sz = 0.0;
     for (int i = tid; i < n; i += 256) {
       if (Z[i] != 0) {
@@ -1140,19 +1264,22 @@ hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
 
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
                       const Inputs& in, const Work& w, bool /*keep_factor*/) {
-  const bool lds = w.N <= kEeqLdsMaxN;
-  size_t shmem = lds ? sizeof(double) * ((size_t)tri((int)w.N) + 3 * w.N + 2 * kWaves)
-                     : sizeof(double) * 2 * kWaves;
-  static size_t configured = 0;
-  if (shmem > configured) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_eeq_chol,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)shmem);
-    if (e != hipSuccess) return e;
-    configured = shmem;
+  if (w.N <= kEeqLdsMaxN) {
+    const size_t shmem = sizeof(double) * ((size_t)tri((int)w.N) + 3 * w.N);
+    static size_t configured = 0;
+    if (shmem > configured) {
+      hipError_t e = hipFuncSetAttribute((const void*)k_eeq_chol<true>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)shmem);
+      if (e != hipSuccess) return e;
+      configured = shmem;
+    }
+    hipLaunchKernelGGL(k_eeq_chol<true>, dim3((unsigned)w.B), dim3(256), shmem, s, t, p,
+                       in, w);
+  } else {
+    hipLaunchKernelGGL(k_eeq_chol<false>, dim3((unsigned)w.B), dim3(256), 0, s, t, p, in,
+                       w);
   }
-  hipLaunchKernelGGL(k_eeq_chol, dim3((unsigned)w.B), dim3(256), shmem, s, t, p, in,
-                     w, lds ? 1 : 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_eeq_lu, dim3((unsigned)w.B), dim3(256), 0, s, t, p, in, w);
