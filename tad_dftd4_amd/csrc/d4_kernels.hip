@@ -27,7 +27,6 @@ constexpr double kCnMax = 8.0;                        // defaults.py:32
 constexpr int kWaves = 4;                             // waves per 256-thread block
 constexpr int kTile = 16;                             // ATM pair tile edge
 constexpr int kZB = 32;                               // ATM third-atom rows per LDS round
-constexpr int kEeqLdsMaxN = 160;                      // EEQ factor in LDS up to this size
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -174,7 +173,7 @@ __global__ __launch_bounds__(256) void k_cn(DevTables t, Params p, Inputs in,
 // EEQ charges: one work-group per molecule.
 //
 // The bordered system [[C, e], [e^T, 0]] [q; mu] = [b; Q] is solved with a
-// Cholesky factorisation of the Coulomb block C (in LDS for n <= 160) and the
+// Cholesky factorisation of the Coulomb block C (in LDS when it fits) and the
 // Schur complement of the border: q = y - mu z with y = C^-1 b, z = C^-1 e,
 // mu = (e.y - Q) / (e.z). A non-positive pivot flags the molecule for the
 // pivoted-LU kernel below.
@@ -197,205 +196,307 @@ __device__ __forceinline__ double eeq_diag(const DevTables& t, int z) {
 
 // ---- blocked Cholesky (16-column blocks) on a packed lower triangle -------
 //
-// Row i of the factor starts at L + tri(i). The diagonal 16 x 16 block is
-// factorised by wave 0 in registers (lane r holds row r; columns travel by
-// cross-lane shuffles, so no barrier is needed inside the block); the panel
-// rows below it are then solved independently by all threads, and the
-// trailing triangle is updated in 2 x 2 register tiles. Three barriers per
-// block instead of three per column.
+// Row i of the matrix starts at L + tri(i). For each block column kb:
+//  1. wave 0 factorises the diagonal 16 x 16 block D in registers (lane r
+//     holds row r; entries are broadcast with v_readlane, no LDS round trip)
+//     and stores X = D^-1 (lower triangular) in its place;
+//  2. every panel row below becomes a X^T (a = the row's block entries) and
+//     the carried right-hand sides become X b: independent per thread;
+//  3. the trailing triangle is updated in register tiles.
+// The substitutions read X from the stored factor in the same way, so each
+// solve step is a small parallel matrix-vector product between two barriers
+// and no thread runs a serial dependent chain longer than 16 terms.
+// All helpers are force-inlined so that pointers into LDS keep their
+// address space (ds_* instructions rather than flat ones).
 
 constexpr int kNB = 16;
 
-// Factor the diagonal block at kb (nb <= 16 rows) with wave 0; writes the
-// block back, stores 1/L_ii in dinv, and returns 1 (on wave 0) if a pivot was
-// not positive.
-__device__ __forceinline__ int chol_diag_block(double* L, int kb, int nb, double* dinv) {
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+// Factor the diagonal block at kb (nb <= 16 rows) on wave 0 and overwrite it
+// with the inverse of its Cholesky factor. Rows nb..15 are padded with the
+// identity. Returns non-zero (uniform over the wave) if a pivot was not
+// positive; the block is then left unspecified.
+template <typename P>
+__device__ __forceinline__ int chol_diag_block(P L, int kb, int nb) {
   const int lane = threadIdx.x & 63;
   double d[kNB];
 #pragma unroll
-  for (int c = 0; c < kNB; ++c)
-    d[c] = (lane < nb && c <= lane) ? L[tri(kb + lane) + kb + c] : 0.0;
+  for (int c = 0; c < kNB; ++c) {
+    double v = (c == lane) ? 1.0 : 0.0;
+    if (lane < nb && c <= lane) v = L[tri(kb + lane) + kb + c];
+    d[c] = v;
+  }
+  double invd[kNB];
   int bad = 0;
 #pragma unroll
   for (int j = 0; j < kNB; ++j) {
-    if (j < nb) {
-      const double piv = __shfl(d[j], j, 64);
-      bad |= !(piv > 0.0);
-      const double sj = sqrt(fmax(piv, 0.0));
-      const double inv = piv > 0.0 ? 1.0 / sj : 0.0;
-      if (lane == j)
-        d[j] = sj;
-      else if (lane > j)
-        d[j] *= inv;
+    const double piv = lane_bcast(d[j], j);
+    bad |= !(piv > 0.0);
+    const double s = piv > 0.0 ? sqrt(piv) : 1.0;
+    const double inv = 1.0 / s;
+    invd[j] = inv;
+    d[j] = lane == j ? s : (lane > j ? d[j] * inv : 0.0);
 #pragma unroll
-      for (int c = j + 1; c < kNB; ++c) {
-        if (c < nb) {
-          const double lcj = __shfl(d[j], c, 64);
-          if (lane >= c) d[c] -= d[j] * lcj;
-        }
-      }
+    for (int c = j + 1; c < kNB; ++c) {
+      const double lcj = lane_bcast(d[j], c);
+      if (lane >= c) d[c] = fma(-d[j], lcj, d[c]);
     }
+  }
+  // lane r forms column r of X = D^-1 by forward substitution
+  double x[kNB];
+#pragma unroll
+  for (int i = 0; i < kNB; ++i) {
+    double s = (i == lane) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) s = fma(-lane_bcast(d[k], i), x[k], s);
+    x[i] = s * invd[i];
   }
   if (lane < nb) {
 #pragma unroll
-    for (int c = 0; c < kNB; ++c)
-      if (c <= lane) L[tri(kb + lane) + kb + c] = d[c];
-    dinv[kb + lane] = 1.0 / d[lane];
+    for (int i = 0; i < kNB; ++i)
+      if (i >= lane && i < nb) L[tri(kb + i) + kb + lane] = x[i];
   }
   return bad;
 }
 
-// In-place row-vector solve v <- v D^-T for the nb x nb lower block D at kb;
-// the result is also copied to out (if non-null). Solved entries are read
-// back from v, which keeps register pressure low.
-__device__ __forceinline__ void block_fsolve(const double* L, int kb, int nb, const double* dinv,
-                                             double* v, double* out) {
-  for (int c = 0; c < nb; ++c) {
-    const double* Lc = L + tri(kb + c) + kb;
-    double sacc = v[c];
-    for (int d = 0; d < c; ++d) sacc -= v[d] * Lc[d];
-    const double x = sacc * dinv[kb + c];
-    v[c] = x;
-    if (out) out[c] = x;
-  }
-}
-
-// Backward solve of one block in place: v_c <- (v_c - sum_{d > c}
-// L[kb+d][kb+c] v_d) / L_cc, copied to out.
-__device__ __forceinline__ void block_bsolve(const double* L, int kb, int nb, const double* dinv,
-                                             double* v, double* out) {
-  for (int c = nb - 1; c >= 0; --c) {
-    double sacc = v[kb + c];
-    for (int d = c + 1; d < nb; ++d) sacc -= L[tri(kb + d) + kb + c] * v[kb + d];
-    const double x = sacc * dinv[kb + c];
-    v[kb + c] = x;
-    out[c] = x;
-  }
-}
-
-// Backward substitution L^T x = y for up to two right-hand sides (v1 may be
-// null), blocked; overwrites v0/v1. Needs 2 * kNB doubles of LDS in yblk.
-__device__ void chol_backward(const double* L, int n, const double* dinv, double* v0,
-                              double* v1, double* yblk) {
-  const int tid = threadIdx.x;
-  for (int kb = ((n - 1) / kNB) * kNB; kb >= 0; kb -= kNB) {
-    const int nb = min(kNB, n - kb);
-    __syncthreads();
-    if (tid == 0) block_bsolve(L, kb, nb, dinv, v0, yblk);
-    if (tid == 1 && v1 != nullptr) block_bsolve(L, kb, nb, dinv, v1, yblk + kNB);
-    __syncthreads();
-    for (int i = tid; i < kb; i += 256) {
-      double s0 = 0.0, s1 = 0.0;
+// Panel row i (full block, nb = 16): L[i][kb..kb+16) <- a X^T.
+template <typename P>
+__device__ __forceinline__ void panel_row(P L, int kb, int i) {
+  P Li = L + tri(i) + kb;
+  double a[kNB];
 #pragma unroll
-      for (int c = 0; c < kNB; ++c) {
-        if (c < nb) {
-          const double l = L[tri(kb + c) + i];
-          s0 += l * yblk[c];
-          s1 += l * yblk[kNB + c];
-        }
+  for (int c = 0; c < kNB; ++c) a[c] = Li[c];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    P Xc = L + tri(kb + c) + kb;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k <= c; ++k) s = fma(a[k], Xc[k], s);
+    Li[c] = s;
+  }
+}
+
+// Block forward step on a vector: v[kb..kb+nb) <- X v[kb..kb+nb), copied to y.
+template <typename P, typename V>
+__device__ __forceinline__ void block_lower(P L, int kb, int nb, V v, double* y) {
+  double a[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) a[c] = c < nb ? v[kb + c] : 0.0;
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) {
+    if (c < nb) {
+      P Xc = L + tri(kb + c) + kb;
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k <= c; ++k) s = fma(a[k], Xc[k], s);
+      v[kb + c] = s;
+      y[c] = s;
+    }
+  }
+}
+
+// Trailing update of one T x T tile: rows r0 + T ti + a, columns r0 + T tj + b
+// (lower part only) lose the product of their panel rows.
+template <int T, typename P>
+__device__ __forceinline__ void trailing_tile(P L, int kb, int r0, int ti, int tj, int n) {
+  const int i0 = r0 + T * ti, j0 = r0 + T * tj;
+  P pi[T], pj[T];
+#pragma unroll
+  for (int a = 0; a < T; ++a) {
+    pi[a] = L + tri(min(i0 + a, n - 1)) + kb;
+    pj[a] = L + tri(min(j0 + a, n - 1)) + kb;
+  }
+  double acc[T][T];
+#pragma unroll
+  for (int a = 0; a < T; ++a)
+#pragma unroll
+    for (int b = 0; b < T; ++b) acc[a][b] = 0.0;
+#pragma unroll 1  // a rolled loop keeps only one chunk of rows in registers
+  for (int c0 = 0; c0 < kNB; c0 += 4) {
+    double xi[T][4], xj[T][4];
+#pragma unroll
+    for (int a = 0; a < T; ++a)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        xi[a][c] = pi[a][c0 + c];
+        xj[a][c] = pj[a][c0 + c];
       }
-      v0[i] -= s0;
-      if (v1 != nullptr) v1[i] -= s1;
-    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int a = 0; a < T; ++a)
+#pragma unroll
+        for (int b = 0; b < T; ++b) acc[a][b] = fma(xi[a][c], xj[b][c], acc[a][b]);
   }
-  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < T; ++a)
+#pragma unroll
+    for (int b = 0; b < T; ++b) {
+      const int i = i0 + a, j = j0 + b;
+      if (i < n && j <= i) L[tri(i) + j] -= acc[a][b];
+    }
 }
 
-// Forward substitution L y = v (one right-hand side), blocked.
-__device__ void chol_forward(const double* L, int n, const double* dinv, double* v,
-                             double* yblk) {
-  const int tid = threadIdx.x;
-  for (int kb = 0; kb < n; kb += kNB) {
-    const int nb = min(kNB, n - kb);
-    __syncthreads();
-    if (tid == 0) block_fsolve(L, kb, nb, dinv, v + kb, yblk);
-    __syncthreads();
-    for (int i = kb + nb + tid; i < n; i += 256) {
-      const double* Li = L + tri(i) + kb;
-      double sacc = 0.0;
+__device__ __forceinline__ void tile_index(int e, int& ti, int& tj) {
+  int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+  while (r * (r + 1) / 2 > e) --r;
+  while ((r + 1) * (r + 2) / 2 <= e) ++r;
+  ti = r;
+  tj = e - r * (r + 1) / 2;
+}
+
+// Trailing triangle (rows and columns r0..n-1) in T x T tiles, plus the
+// update of the two carried right-hand sides by the block's solution y.
+template <int T, typename P, typename V>
+__device__ __forceinline__ void trailing_update(P L, int kb, int n, V rb, V re,
+                                                const double* yblk) {
+  const int r0 = kb + kNB;
+  const int mr = n - r0;
+  const int mt = (mr + T - 1) / T;
+  const int ntl = mt * (mt + 1) / 2;
+  // right-hand-side rows first, on the threads that the tiles leave idle last
+  const int first = (256 - (ntl + mr) % 256) % 256;
+  for (int e = threadIdx.x - first; e < mr; e += 256) {
+    if (e < 0) continue;
+    const int i = r0 + e;
+    P Pi = L + tri(i) + kb;
+    double sb = 0.0, se = 0.0;
 #pragma unroll
-      for (int c = 0; c < kNB; ++c)
-        if (c < nb) sacc += Li[c] * yblk[c];
-      v[i] -= sacc;
+    for (int c = 0; c < kNB; ++c) {
+      sb = fma(Pi[c], yblk[c], sb);
+      se = fma(Pi[c], yblk[kNB + c], se);
     }
+    rb[i] -= sb;
+    re[i] -= se;
   }
-  __syncthreads();
+  for (int e = threadIdx.x; e < ntl; e += 256) {
+    int ti, tj;
+    tile_index(e, ti, tj);
+    trailing_tile<T>(L, kb, r0, ti, tj, n);
+  }
 }
 
 // Blocked right-looking Cholesky of the assembled n x n lower triangle with
 // two right-hand sides carried along (forward substitution fused): on return
-// rb, re hold L^-1 b and L^-1 e. Returns non-zero if a pivot was not positive.
-__device__ int chol_factor(double* L, int n, double* dinv, double* rb, double* re,
-                           double* yblk, double* flag) {
+// rb, re hold L^-1 b and L^-1 e and the diagonal blocks hold their inverses.
+// Returns non-zero if a pivot was not positive.
+template <typename P, typename V>
+__device__ __forceinline__ int chol_factor(P L, int n, V rb, V re, double* yblk,
+                                           double* flag) {
   const int tid = threadIdx.x;
   for (int kb = 0; kb < n; kb += kNB) {
     const int nb = min(kNB, n - kb);
     if (tid < 64) {
-      const int bad = chol_diag_block(L, kb, nb, dinv);
+      const int bad = chol_diag_block(L, kb, nb);
       if (tid == 0) flag[0] = bad ? 1.0 : 0.0;
     }
     __syncthreads();
     if (flag[0] != 0.0) return 1;  // uniform
-    const int mr = n - kb - nb;
-    // panel rows and the two right-hand-side blocks
+    const int mr = n - kb - nb;    // > 0 only for full blocks
     for (int t2 = tid; t2 < mr + 2; t2 += 256) {
       if (t2 < mr)
-        block_fsolve(L, kb, nb, dinv, L + tri(kb + nb + t2) + kb, nullptr);
+        panel_row(L, kb, kb + nb + t2);
       else
-        block_fsolve(L, kb, nb, dinv, (t2 == mr ? rb : re) + kb, yblk + (t2 - mr) * kNB);
+        block_lower(L, kb, nb, t2 == mr ? rb : re, yblk + (t2 - mr) * kNB);
     }
     __syncthreads();
-    // trailing update in 2 x 2 tiles, plus the right-hand sides
-    const int mt = (mr + 1) / 2;
-    const int ntl = mt * (mt + 1) / 2;
-    for (int e = tid; e < ntl + mr; e += 256) {
-      if (e < ntl) {
-        int ti = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-        while (ti * (ti + 1) / 2 > e) --ti;
-        while ((ti + 1) * (ti + 2) / 2 <= e) ++ti;
-        const int tj = e - ti * (ti + 1) / 2;
-        const int i0 = kb + nb + 2 * ti, j0 = kb + nb + 2 * tj;
-        const bool i1 = i0 + 1 < n, j1 = j0 + 1 < n;
-        const double* Pi0 = L + tri(i0) + kb;
-        const double* Pi1 = i1 ? L + tri(i0 + 1) + kb : Pi0;
-        const double* Pj0 = L + tri(j0) + kb;
-        const double* Pj1 = j1 ? L + tri(j0 + 1) + kb : Pj0;
-        double a00 = 0.0, a10 = 0.0, a01 = 0.0, a11 = 0.0;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-          if (c < nb) {
-            const double x0 = Pi0[c], x1 = Pi1[c], y0 = Pj0[c], y1 = Pj1[c];
-            a00 += x0 * y0;
-            a10 += x1 * y0;
-            a01 += x0 * y1;
-            a11 += x1 * y1;
-          }
-        }
-        L[tri(i0) + j0] -= a00;
-        if (i1) L[tri(i0 + 1) + j0] -= a10;
-        if (j1 && j0 + 1 <= i0) L[tri(i0) + j0 + 1] -= a01;
-        if (i1 && j1) L[tri(i0 + 1) + j0 + 1] -= a11;
-      } else {
-        const int i = kb + nb + (e - ntl);
-        const double* Pi = L + tri(i) + kb;
-        double sb = 0.0, se = 0.0;
-#pragma unroll
-        for (int c = 0; c < kNB; ++c)
-          if (c < nb) {
-            sb += Pi[c] * yblk[c];
-            se += Pi[c] * yblk[kNB + c];
-          }
-        rb[i] -= sb;
-        re[i] -= se;
-      }
+    if (mr > 0) {
+      if (mr >= 48)
+        trailing_update<4>(L, kb, n, rb, re, yblk);
+      else
+        trailing_update<2>(L, kb, n, rb, re, yblk);
     }
     __syncthreads();
   }
   return 0;
 }
 
-template <bool LDS>
-__global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p, Inputs in,
+// Backward substitution L^T x = y for one or two right-hand sides (v1 may be
+// null) with the stored factor; overwrites v0 / v1. yblk: 2 * kNB doubles.
+template <typename P, typename V>
+__device__ __forceinline__ void chol_backward(P L, int n, V v0, V v1, double* yblk) {
+  const int tid = threadIdx.x;
+  const int c = tid & (kNB - 1), r = tid >> 4;  // r < 2 for the block step
+  for (int kb = ((n - 1) / kNB) * kNB; kb >= 0; kb -= kNB) {
+    const int nb = min(kNB, n - kb);
+    if (tid < 2 * kNB && c < nb && (r == 0 || v1 != nullptr)) {
+      V v = r == 0 ? v0 : v1;
+      double s = 0.0;
+      for (int k = c; k < nb; ++k) s = fma(L[tri(kb + k) + kb + c], v[kb + k], s);
+      yblk[r * kNB + c] = s;
+    }
+    __syncthreads();
+    if (tid < 2 * kNB && c < nb && (r == 0 || v1 != nullptr))
+      (r == 0 ? v0 : v1)[kb + c] = yblk[r * kNB + c];
+    for (int i = tid; i < kb; i += 256) {
+      double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+      for (int cc = 0; cc < kNB; ++cc) {
+        if (cc < nb) {  // the last block may be partial
+          const double l = L[tri(kb + cc) + i];
+          s0 = fma(l, yblk[cc], s0);
+          s1 = fma(l, yblk[kNB + cc], s1);
+        }
+      }
+      v0[i] -= s0;
+      if (v1 != nullptr) v1[i] -= s1;
+    }
+    __syncthreads();
+  }
+}
+
+// Forward substitution L y = v (one right-hand side) with the stored factor.
+template <typename P, typename V>
+__device__ __forceinline__ void chol_forward(P L, int n, V v, double* yblk) {
+  const int tid = threadIdx.x;
+  for (int kb = 0; kb < n; kb += kNB) {
+    const int nb = min(kNB, n - kb);
+    if (tid < nb) {
+      P Xc = L + tri(kb + tid) + kb;
+      double s = 0.0;
+      for (int k = 0; k <= tid; ++k) s = fma(Xc[k], v[kb + k], s);
+      yblk[tid] = s;
+    }
+    __syncthreads();
+    if (tid < nb) v[kb + tid] = yblk[tid];
+    for (int i = kb + nb + tid; i < n; i += 256) {
+      P Li = L + tri(i) + kb;
+      double s = 0.0;
+#pragma unroll
+      for (int cc = 0; cc < kNB; ++cc) s = fma(Li[cc], yblk[cc], s);
+      v[i] -= s;
+    }
+    __syncthreads();
+  }
+}
+
+// LDS placement of the EEQ solve for a padded length N: the packed factor
+// and the two vectors (kEeqAllLds), the factor only (kEeqFactorLds), or
+// nothing (kEeqGlobal, HBM workspace).
+constexpr int kEeqGlobal = 0, kEeqFactorLds = 1, kEeqAllLds = 2;
+constexpr size_t kLdsBytes = 160 * 1024 - 1024;  // dynamic LDS budget per work-group
+
+inline int eeq_mode(int64_t N, int vectors) {
+  const size_t f = sizeof(double) * (size_t)tri((int)N);
+  if (f + sizeof(double) * vectors * (size_t)N <= kLdsBytes) return kEeqAllLds;
+  if (f <= kLdsBytes) return kEeqFactorLds;
+  return kEeqGlobal;
+}
+
+inline size_t eeq_shmem(int mode, int64_t N, int vectors) {
+  if (mode == kEeqGlobal) return 0;
+  return sizeof(double) * ((size_t)tri((int)N) + (mode == kEeqAllLds ? vectors * N : 0));
+}
+
+// Two work-groups per CU fit the LDS of a 120-atom factor; keep the register
+// use low enough for two waves per SIMD as well.
+template <int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eeq_chol(DevTables t, Params p, Inputs in,
                                                   Work w) {
   extern __shared__ double smem[];
   __shared__ double s_small[2 * kNB + 2 * kWaves + 2];
@@ -415,10 +516,11 @@ __global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p, Inputs 
   for (int i = n + tid; i < N; i += 256) q[i] = 0.0;
   if (n == 0) return;
 
-  double* L = LDS ? smem : w.chol + (size_t)m * (size_t)tri((int)N);
-  double* rb = LDS ? smem + (size_t)tri((int)N) : w.lam + m * N;  // scratch until the adjoint
-  double* re = LDS ? rb + N : w.ez + m * N;
-  double* dinv = LDS ? re + N : w.beta + m * N;  // scratch until the adjoint
+  double* gL = w.chol + (size_t)m * (size_t)tri((int)N);
+  double* L = MODE == kEeqGlobal ? gL : smem;
+  // rb is scratch in lam until the adjoint; re (= C^-1 e) is kept in ez
+  double* rb = MODE == kEeqAllLds ? smem + tri((int)N) : w.lam + m * N;
+  double* re = MODE == kEeqAllLds ? rb + N : w.ez + m * N;
   double* yblk = s_small;
   double* red = s_small + 2 * kNB;
   double* flag = red + 2 * kWaves;
@@ -448,11 +550,11 @@ __global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p, Inputs 
   }
   __syncthreads();
 
-  if (p.force_lu || chol_factor(L, n, dinv, rb, re, yblk, flag)) {
+  if (p.force_lu || chol_factor(L, n, rb, re, yblk, flag)) {
     if (tid == 0) w.status[m] |= kStatusLU;
     return;
   }
-  chol_backward(L, n, dinv, rb, re, yblk);
+  chol_backward(L, n, rb, re, yblk);
 
   double sb = 0.0, se = 0.0;
   for (int i = tid; i < n; i += 256) {
@@ -465,11 +567,11 @@ __global__ __launch_bounds__(256) void k_eeq_chol(DevTables t, Params p, Inputs 
   se = block_sum(se, red + kWaves);
   const double mu = (sb - in.charge[m]) / se;
   for (int i = tid; i < n; i += 256) q[i] = Z[i] != 0 ? rb[i] - mu * re[i] : 0.0;
-  if (LDS) {
-    // keep z = C^-1 e and the factor for the adjoint solve
+  if (MODE == kEeqAllLds) {
     double* gz = w.ez + m * N;
     for (int i = tid; i < n; i += 256) gz[i] = re[i];
-    double* gL = w.chol + (size_t)m * (size_t)tri((int)N);
+  }
+  if (MODE != kEeqGlobal) {
     const int nt = tri(n);
     for (int k = tid; k < nt; k += 256) gL[k] = L[k];
   }
@@ -1064,8 +1166,10 @@ __global__ __launch_bounds__(256) void k_atm_rows(Inputs in, Work w,
 // beta_i = lambda_i d b_i / d raw-CN_i. One work-group per molecule.
 // --------------------------------------------------------------------------
 
+template <int MODE>
 __global__ __launch_bounds__(256) void k_adjoint(DevTables t, Params p, Inputs in,
                                                  Work w) {
+  extern __shared__ double smem[];
   __shared__ double red[2 * kWaves];
   __shared__ double yblk[2 * kNB];
   const int64_t m = blockIdx.x;
@@ -1097,28 +1201,30 @@ __global__ __launch_bounds__(256) void k_adjoint(DevTables t, Params p, Inputs i
     lu_solve(A, ld, piv, n + 1, vec);
     for (int i = tid; i < n; i += 256) lam[i] = Z[i] != 0 ? vec[i] : 0.0;
   } else {
-    const double* L = w.chol + (size_t)m * (size_t)tri((int)N);
+    double* gL = w.chol + (size_t)m * (size_t)tri((int)N);
+    double* L = MODE == kEeqGlobal ? gL : smem;
+    double* v = MODE == kEeqAllLds ? smem + tri((int)N) : lam;
     const double* ez = w.ez + m * N;
-    double* dinv = w.beta + m * N;  // scratch: 1/L_ii, beta is written last
-    for (int i = tid; i < n; i += 256) {
-      lam[i] = Z[i] != 0 ? g[i] : 0.0;
-      dinv[i] = 1.0 / L[tri(i) + i];
+    if (MODE != kEeqGlobal) {
+      const int nt = tri(n);
+      for (int k = tid; k < nt; k += 256) L[k] = gL[k];
     }
-    // w = C^-1 g in lam
-    chol_forward(L, n, dinv, lam, yblk);
-    chol_backward(L, n, dinv, lam, nullptr, yblk);
+    for (int i = tid; i < n; i += 256) v[i] = Z[i] != 0 ? g[i] : 0.0;
+    __syncthreads();
+    // w = C^-1 g
+    chol_forward(L, n, v, yblk);
+    chol_backward(L, n, v, (double*)nullptr, yblk);
     double sw = 0.0, sz = 0.0;
     for (int i = tid; i < n; i += 256) {
       if (Z[i] != 0) {
-        sw += lam[i];
+        sw += v[i];
         sz += ez[i];
       }
     }
     sw = block_sum(sw, red);
     sz = block_sum(sz, red + kWaves);
     const double nu = sw / sz;
-    __syncthreads();
-    for (int i = tid; i < n; i += 256) lam[i] = Z[i] != 0 ? lam[i] - nu * ez[i] : 0.0;
+    for (int i = tid; i < n; i += 256) lam[i] = Z[i] != 0 ? v[i] - nu * ez[i] : 0.0;
   }
   __syncthreads();
   for (int i = tid; i < n; i += 256) {
@@ -1262,27 +1368,36 @@ hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
   return hipGetLastError();
 }
 
+// Launch a per-molecule EEQ kernel with `shmem` bytes of dynamic LDS, raising
+// the kernel's LDS limit first when needed (once per kernel and size).
+template <typename... Args>
+hipError_t launch_lds(void (*kernel)(Args...), size_t& configured, unsigned grid,
+                      size_t shmem, hipStream_t s, Args... args) {
+  if (shmem > configured) {
+    hipError_t e = hipFuncSetAttribute((const void*)kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+    configured = shmem;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), shmem, s, args...);
+  return hipGetLastError();
+}
+
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
                       const Inputs& in, const Work& w, bool /*keep_factor*/) {
-  if (w.N <= kEeqLdsMaxN) {
-    const size_t shmem = sizeof(double) * ((size_t)tri((int)w.N) + 3 * w.N);
-    static size_t configured = 0;
-    if (shmem > configured) {
-      hipError_t e = hipFuncSetAttribute((const void*)k_eeq_chol<true>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)shmem);
-      if (e != hipSuccess) return e;
-      configured = shmem;
-    }
-    hipLaunchKernelGGL(k_eeq_chol<true>, dim3((unsigned)w.B), dim3(256), shmem, s, t, p,
-                       in, w);
-  } else {
-    hipLaunchKernelGGL(k_eeq_chol<false>, dim3((unsigned)w.B), dim3(256), 0, s, t, p, in,
-                       w);
-  }
-  hipError_t e = hipGetLastError();
+  static size_t configured[3] = {0, 0, 0};
+  const int mode = eeq_mode(w.N, 2);
+  const size_t shmem = eeq_shmem(mode, w.N, 2);
+  const unsigned grid = (unsigned)w.B;
+  hipError_t e;
+  if (mode == kEeqAllLds)
+    e = launch_lds(k_eeq_chol<kEeqAllLds>, configured[2], grid, shmem, s, t, p, in, w);
+  else if (mode == kEeqFactorLds)
+    e = launch_lds(k_eeq_chol<kEeqFactorLds>, configured[1], grid, shmem, s, t, p, in, w);
+  else
+    e = launch_lds(k_eeq_chol<kEeqGlobal>, configured[0], grid, shmem, s, t, p, in, w);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_eeq_lu, dim3((unsigned)w.B), dim3(256), 0, s, t, p, in, w);
+  hipLaunchKernelGGL(k_eeq_lu, dim3(grid), dim3(256), 0, s, t, p, in, w);
   return hipGetLastError();
 }
 
@@ -1341,8 +1456,15 @@ hipError_t launch_atm_rows(hipStream_t s, const Inputs& in, const Work& w,
 
 hipError_t launch_adjoint(hipStream_t s, const DevTables& t, const Params& p,
                           const Inputs& in, const Work& w) {
-  hipLaunchKernelGGL(k_adjoint, dim3((unsigned)w.B), dim3(256), 0, s, t, p, in, w);
-  return hipGetLastError();
+  static size_t configured[3] = {0, 0, 0};
+  const int mode = eeq_mode(w.N, 1);
+  const size_t shmem = eeq_shmem(mode, w.N, 1);
+  const unsigned grid = (unsigned)w.B;
+  if (mode == kEeqAllLds)
+    return launch_lds(k_adjoint<kEeqAllLds>, configured[2], grid, shmem, s, t, p, in, w);
+  if (mode == kEeqFactorLds)
+    return launch_lds(k_adjoint<kEeqFactorLds>, configured[1], grid, shmem, s, t, p, in, w);
+  return launch_lds(k_adjoint<kEeqGlobal>, configured[0], grid, shmem, s, t, p, in, w);
 }
 
 hipError_t launch_chain(hipStream_t s, const DevTables& t, const Params& p,
