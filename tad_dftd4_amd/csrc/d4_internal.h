@@ -67,7 +67,7 @@ struct Work {
   double* dgw0;     // [B*N*8] d gw0 / d CN
   double* dgwq;     // [B*N*8] d gw / d q
   double* chol;     // [B*NT] packed lower Cholesky factor of the Coulomb block
-  double* ez;       // [B*N] C^-1 e (e = 1 on real atoms)
+  double* ez;       // [B*N] y_e = L^-1 e (forward-substituted border, e = 1 on real atoms)
   double* lu;       // [B*(N+1)^2] pivoted LU of the bordered matrix (fallback)
   int32_t* piv;     // [B*(N+1)]
   double* vec;      // [B*(N+1)] right-hand side / solution of the LU path

@@ -219,7 +219,8 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 
 // Factor the diagonal block at kb (nb <= 16 rows) on wave 0 and overwrite it
 // with the inverse of its Cholesky factor. Rows nb..15 are padded with the
-// identity. Returns non-zero (uniform over the wave) if a pivot was not
+// identity (their columns never mix with the real rows: their off-diagonal
+// entries are zero and stay zero). Returns non-zero (uniform over the wave) if a pivot was not
 // positive; the block is then left unspecified.
 template <typename P>
 __device__ __forceinline__ int chol_diag_block(P L, int kb, int nb) {
@@ -240,12 +241,12 @@ __device__ __forceinline__ int chol_diag_block(P L, int kb, int nb) {
     const double s = piv > 0.0 ? sqrt(piv) : 1.0;
     const double inv = 1.0 / s;
     invd[j] = inv;
-    d[j] = lane == j ? s : (lane > j ? d[j] * inv : 0.0);
+    // Lanes above the diagonal (lane < j) carry junk from here on in d[j] and
+    // in the columns they update; only lanes >= column are ever broadcast
+    // (pivots and L[c][j] with c > j), so the junk is never read.
+    d[j] = lane == j ? s : d[j] * inv;
 #pragma unroll
-    for (int c = j + 1; c < kNB; ++c) {
-      const double lcj = lane_bcast(d[j], c);
-      if (lane >= c) d[c] = fma(-d[j], lcj, d[c]);
-    }
+    for (int c = j + 1; c < kNB; ++c) d[c] = fma(-d[j], lane_bcast(d[j], c), d[c]);
   }
   // lane r forms column r of X = D^-1 by forward substitution
   double x[kNB];
@@ -384,6 +385,7 @@ __device__ __forceinline__ void trailing_update(P L, int kb, int n, V rb, V re,
 // Blocked right-looking Cholesky of the assembled n x n lower triangle with
 // two right-hand sides carried along (forward substitution fused): on return
 // rb, re hold L^-1 b and L^-1 e and the diagonal blocks hold their inverses.
+// yblk: 2 * kNB doubles of LDS.
 // Returns non-zero if a pivot was not positive.
 template <typename P, typename V>
 __device__ __forceinline__ int chol_factor(P L, int n, V rb, V re, double* yblk,
@@ -416,50 +418,45 @@ __device__ __forceinline__ int chol_factor(P L, int n, V rb, V re, double* yblk,
   return 0;
 }
 
-// Backward substitution L^T x = y for one or two right-hand sides (v1 may be
-// null) with the stored factor; overwrites v0 / v1. yblk: 2 * kNB doubles.
+// Backward substitution L^T x = y with the stored factor; overwrites v.
+// yblk: kNB doubles of LDS.
 template <typename P, typename V>
-__device__ __forceinline__ void chol_backward(P L, int n, V v0, V v1, double* yblk) {
+__device__ __forceinline__ void chol_backward(P L, int n, V v, double* yblk) {
   const int tid = threadIdx.x;
-  const int c = tid & (kNB - 1), r = tid >> 4;  // r < 2 for the block step
   for (int kb = ((n - 1) / kNB) * kNB; kb >= 0; kb -= kNB) {
     const int nb = min(kNB, n - kb);
-    if (tid < 2 * kNB && c < nb && (r == 0 || v1 != nullptr)) {
-      V v = r == 0 ? v0 : v1;
+    if (tid < nb) {  // x_c = sum_{k >= c} X[k][c] v_k
       double s = 0.0;
-      for (int k = c; k < nb; ++k) s = fma(L[tri(kb + k) + kb + c], v[kb + k], s);
-      yblk[r * kNB + c] = s;
+#pragma unroll
+      for (int k = 0; k < kNB; ++k)
+        if (k >= tid && k < nb) s = fma(L[tri(kb + k) + kb + tid], v[kb + k], s);
+      yblk[tid] = s;
     }
     __syncthreads();
-    if (tid < 2 * kNB && c < nb && (r == 0 || v1 != nullptr))
-      (r == 0 ? v0 : v1)[kb + c] = yblk[r * kNB + c];
+    if (tid < nb) v[kb + tid] = yblk[tid];
     for (int i = tid; i < kb; i += 256) {
-      double s0 = 0.0, s1 = 0.0;
+      double s = 0.0;
 #pragma unroll
-      for (int cc = 0; cc < kNB; ++cc) {
-        if (cc < nb) {  // the last block may be partial
-          const double l = L[tri(kb + cc) + i];
-          s0 = fma(l, yblk[cc], s0);
-          s1 = fma(l, yblk[kNB + cc], s1);
-        }
-      }
-      v0[i] -= s0;
-      if (v1 != nullptr) v1[i] -= s1;
+      for (int cc = 0; cc < kNB; ++cc)
+        if (cc < nb) s = fma(L[tri(kb + cc) + i], yblk[cc], s);  // last block may be partial
+      v[i] -= s;
     }
     __syncthreads();
   }
 }
 
-// Forward substitution L y = v (one right-hand side) with the stored factor.
+// Forward substitution L y = v with the stored factor; overwrites v.
 template <typename P, typename V>
 __device__ __forceinline__ void chol_forward(P L, int n, V v, double* yblk) {
   const int tid = threadIdx.x;
   for (int kb = 0; kb < n; kb += kNB) {
     const int nb = min(kNB, n - kb);
-    if (tid < nb) {
+    if (tid < nb) {  // y_c = sum_{k <= c} X[c][k] v_k
       P Xc = L + tri(kb + tid) + kb;
       double s = 0.0;
-      for (int k = 0; k <= tid; ++k) s = fma(Xc[k], v[kb + k], s);
+#pragma unroll
+      for (int k = 0; k < kNB; ++k)
+        if (k <= tid) s = fma(Xc[k], v[kb + k], s);
       yblk[tid] = s;
     }
     __syncthreads();
@@ -476,10 +473,11 @@ __device__ __forceinline__ void chol_forward(P L, int n, V v, double* yblk) {
 }
 
 // LDS placement of the EEQ solve for a padded length N: the packed factor
-// and the two vectors (kEeqAllLds), the factor only (kEeqFactorLds), or
-// nothing (kEeqGlobal, HBM workspace).
+// and `vectors` per-atom vectors (kEeqAllLds), the factor only
+// (kEeqFactorLds), or nothing (kEeqGlobal, HBM workspace).
 constexpr int kEeqGlobal = 0, kEeqFactorLds = 1, kEeqAllLds = 2;
 constexpr size_t kLdsBytes = 160 * 1024 - 1024;  // dynamic LDS budget per work-group
+constexpr int kEeqVectors = 6;                   // y_b, y_e and x, y, z, radius of each atom
 
 inline int eeq_mode(int64_t N, int vectors) {
   const size_t f = sizeof(double) * (size_t)tri((int)N);
@@ -493,13 +491,21 @@ inline size_t eeq_shmem(int mode, int64_t N, int vectors) {
   return sizeof(double) * ((size_t)tri((int)N) + (mode == kEeqAllLds ? vectors * N : 0));
 }
 
+// Row of a strictly-lower packed index e: i (i - 1) / 2 <= e < i (i + 1) / 2.
+__device__ __forceinline__ int strict_row(int e) {
+  int i = (int)((sqrtf(8.0f * (float)e + 1.0f) + 1.0f) * 0.5f);
+  while (i * (i - 1) / 2 > e) --i;
+  while (i * (i + 1) / 2 <= e) ++i;
+  return i;
+}
+
 // Two work-groups per CU fit the LDS of a 120-atom factor; keep the register
 // use low enough for two waves per SIMD as well.
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eeq_chol(DevTables t, Params p, Inputs in,
-                                                  Work w) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eeq_chol(
+    DevTables t, Params p, Inputs in, Work w) {
   extern __shared__ double smem[];
-  __shared__ double s_small[2 * kNB + 2 * kWaves + 2];
+  __shared__ double s_small[2 * kNB + 2 * kWaves + 2];  // two solution blocks, sums, flag
   const int64_t m = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t N = w.N;
@@ -518,7 +524,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
   double* gL = w.chol + (size_t)m * (size_t)tri((int)N);
   double* L = MODE == kEeqGlobal ? gL : smem;
-  // rb is scratch in lam until the adjoint; re (= C^-1 e) is kept in ez
+  // y_b = L^-1 b is scratch in lam until the adjoint; y_e = L^-1 e is kept in ez
   double* rb = MODE == kEeqAllLds ? smem + tri((int)N) : w.lam + m * N;
   double* re = MODE == kEeqAllLds ? rb + N : w.ez + m * N;
   double* yblk = s_small;
@@ -526,26 +532,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   double* flag = red + 2 * kWaves;
 
   // assemble the lower triangle of C and the two right-hand sides
-  for (int i = wave; i < n; i += kWaves) {
-    const int zi = (int)Z[i];
-    const double xi = R[3 * i], yi = R[3 * i + 1], zzi = R[3 * i + 2];
-    double* Li = L + tri(i);
-    for (int j = lane; j <= i; j += 64) {
-      const int zj = (int)Z[j];
-      double v;
-      if (j == i) {
-        v = zi != 0 ? eeq_diag(t, zi) : 1.0;
-      } else if (zi != 0 && zj != 0) {
-        const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
-        v = eeq_offdiag(t, zi, zj, sqrt(dx * dx + dy * dy + dz * dz));
-      } else {
-        v = 0.0;
-      }
-      Li[j] = v;
-    }
-    if (lane == 0) {
+  if (MODE == kEeqAllLds) {
+    // stage positions and radii (0 marks a padding atom), then fill the
+    // strictly-lower triangle with all threads
+    double* sx = re + N;
+    double* sy = sx + N;
+    double* sz = sy + N;
+    double* srad = sz + N;
+    for (int i = tid; i < n; i += 256) {
+      const int zi = (int)Z[i];
+      sx[i] = R[3 * i];
+      sy[i] = R[3 * i + 1];
+      sz[i] = R[3 * i + 2];
+      srad[i] = zi != 0 ? t.rad[zi] : 0.0;
+      L[tri(i) + i] = zi != 0 ? eeq_diag(t, zi) : 1.0;
       rb[i] = zi != 0 ? eeq_rhs(t, zi, w.cnq[m * N + i]) : 0.0;
       re[i] = zi != 0 ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int ne = n * (n - 1) / 2;
+    for (int e = tid; e < ne; e += 256) {
+      const int i = strict_row(e);
+      const int j = e - i * (i - 1) / 2;
+      const double ri = srad[i], rj = srad[j];
+      double v = 0.0;
+      if (ri != 0.0 && rj != 0.0) {
+        const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        v = erf(r / sqrt(ri * ri + rj * rj)) / r;
+      }
+      L[tri(i) + j] = v;
+    }
+  } else {
+    for (int i = wave; i < n; i += kWaves) {
+      const int zi = (int)Z[i];
+      const double xi = R[3 * i], yi = R[3 * i + 1], zzi = R[3 * i + 2];
+      double* Li = L + tri(i);
+      for (int j = lane; j <= i; j += 64) {
+        const int zj = (int)Z[j];
+        double v;
+        if (j == i) {
+          v = zi != 0 ? eeq_diag(t, zi) : 1.0;
+        } else if (zi != 0 && zj != 0) {
+          const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
+          v = eeq_offdiag(t, zi, zj, sqrt(dx * dx + dy * dy + dz * dz));
+        } else {
+          v = 0.0;
+        }
+        Li[j] = v;
+      }
+      if (lane == 0) {
+        rb[i] = zi != 0 ? eeq_rhs(t, zi, w.cnq[m * N + i]) : 0.0;
+        re[i] = zi != 0 ? 1.0 : 0.0;
+      }
     }
   }
   __syncthreads();
@@ -554,19 +593,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     if (tid == 0) w.status[m] |= kStatusLU;
     return;
   }
-  chol_backward(L, n, rb, re, yblk);
-
-  double sb = 0.0, se = 0.0;
+  // e^T C^-1 b = y_e . y_b and e^T C^-1 e = y_e . y_e, so one backward
+  // substitution of y_b - mu y_e gives q = C^-1 (b - mu e)
+  double sbe = 0.0, see = 0.0;
   for (int i = tid; i < n; i += 256) {
-    if (Z[i] != 0) {
-      sb += rb[i];
-      se += re[i];
-    }
+    sbe = fma(re[i], rb[i], sbe);
+    see = fma(re[i], re[i], see);
   }
-  sb = block_sum(sb, red);
-  se = block_sum(se, red + kWaves);
-  const double mu = (sb - in.charge[m]) / se;
-  for (int i = tid; i < n; i += 256) q[i] = Z[i] != 0 ? rb[i] - mu * re[i] : 0.0;
+  sbe = block_sum(sbe, red);
+  see = block_sum(see, red + kWaves);
+  const double mu = (sbe - in.charge[m]) / see;
+  for (int i = tid; i < n; i += 256) rb[i] = fma(-mu, re[i], rb[i]);
+  __syncthreads();
+  chol_backward(L, n, rb, yblk);
+  for (int i = tid; i < n; i += 256) q[i] = Z[i] != 0 ? rb[i] : 0.0;
   if (MODE == kEeqAllLds) {
     double* gz = w.ez + m * N;
     for (int i = tid; i < n; i += 256) gz[i] = re[i];
@@ -1171,7 +1211,7 @@ __global__ __launch_bounds__(256) void k_adjoint(DevTables t, Params p, Inputs i
                                                  Work w) {
   extern __shared__ double smem[];
   __shared__ double red[2 * kWaves];
-  __shared__ double yblk[2 * kNB];
+  __shared__ double yblk[kNB];
   const int64_t m = blockIdx.x;
   const int64_t N = w.N;
   const int n = w.natom[m];
@@ -1204,27 +1244,28 @@ __global__ __launch_bounds__(256) void k_adjoint(DevTables t, Params p, Inputs i
     double* gL = w.chol + (size_t)m * (size_t)tri((int)N);
     double* L = MODE == kEeqGlobal ? gL : smem;
     double* v = MODE == kEeqAllLds ? smem + tri((int)N) : lam;
-    const double* ez = w.ez + m * N;
+    const double* ye = w.ez + m * N;
     if (MODE != kEeqGlobal) {
       const int nt = tri(n);
       for (int k = tid; k < nt; k += 256) L[k] = gL[k];
     }
     for (int i = tid; i < n; i += 256) v[i] = Z[i] != 0 ? g[i] : 0.0;
     __syncthreads();
-    // w = C^-1 g
+    // lambda = C^-1 (g - nu e), nu = e^T C^-1 g / e^T C^-1 e, with the
+    // forward-substituted y_e = L^-1 e kept by the charge kernel
     chol_forward(L, n, v, yblk);
-    chol_backward(L, n, v, (double*)nullptr, yblk);
     double sw = 0.0, sz = 0.0;
     for (int i = tid; i < n; i += 256) {
-      if (Z[i] != 0) {
-        sw += v[i];
-        sz += ez[i];
-      }
+      sw = fma(ye[i], v[i], sw);
+      sz = fma(ye[i], ye[i], sz);
     }
     sw = block_sum(sw, red);
     sz = block_sum(sz, red + kWaves);
     const double nu = sw / sz;
-    for (int i = tid; i < n; i += 256) lam[i] = Z[i] != 0 ? v[i] - nu * ez[i] : 0.0;
+    for (int i = tid; i < n; i += 256) v[i] = fma(-nu, ye[i], v[i]);
+    __syncthreads();
+    chol_backward(L, n, v, yblk);
+    for (int i = tid; i < n; i += 256) lam[i] = Z[i] != 0 ? v[i] : 0.0;
   }
   __syncthreads();
   for (int i = tid; i < n; i += 256) {
@@ -1386,8 +1427,8 @@ hipError_t launch_lds(void (*kernel)(Args...), size_t& configured, unsigned grid
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
                       const Inputs& in, const Work& w, bool /*keep_factor*/) {
   static size_t configured[3] = {0, 0, 0};
-  const int mode = eeq_mode(w.N, 2);
-  const size_t shmem = eeq_shmem(mode, w.N, 2);
+  const int mode = eeq_mode(w.N, kEeqVectors);
+  const size_t shmem = eeq_shmem(mode, w.N, kEeqVectors);
   const unsigned grid = (unsigned)w.B;
   hipError_t e;
   if (mode == kEeqAllLds)
