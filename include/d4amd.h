@@ -37,7 +37,8 @@ extern "C" {
 
 /* d4_params.flags */
 #define D4_FLAG_S10 1      /* "s10" key present: evaluate the order-10 term */
-#define D4_FLAG_FORCE_LU 2 /* testing: solve every EEQ system by pivoted LU */
+#define D4_FLAG_FORCE_LU 2    /* testing: solve every EEQ system by pivoted LU */
+#define D4_FLAG_FORCE_TILED 4 /* testing: use the large-system (tiled) EEQ path at any size */
 
 /* Element and reference-system tables, host memory, read once by
  * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference

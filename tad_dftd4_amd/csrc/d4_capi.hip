@@ -64,6 +64,7 @@ d4::Params to_params(const d4_params* p) {
   k.cuteeq = p->cutoff_eeq;
   k.s10on = (p->flags & D4_FLAG_S10) ? 1 : 0;
   k.force_lu = (p->flags & D4_FLAG_FORCE_LU) ? 1 : 0;
+  k.force_tiled = (p->flags & D4_FLAG_FORCE_TILED) ? 1 : 0;
   return k;
 }
 

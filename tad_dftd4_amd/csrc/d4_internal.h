@@ -46,6 +46,7 @@ struct Params {
   double cut2, cut3sq, cutcn, cuteeq;
   int s10on;
   int force_lu;
+  int force_tiled;
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).

@@ -180,7 +180,8 @@ def test_forced_lu_path_matches_cholesky():
 
 
 def test_large_molecule_global_eeq():
-    """n = 200 exceeds the LDS Cholesky size: the factor lives in HBM."""
+    """n = 200: the packed factor fills the LDS, the solution vectors live in
+    HBM."""
     _say("n=200 molecule")
     mol = _mol(40, 200)
     eo, go = _oracle(mol)
