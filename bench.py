@@ -2,19 +2,27 @@
 """
 Benchmark of the D4 energy + forces path on one MI355X.
 
-Workload ``batch_1024_20_120``: 1024 synthetic molecules, n ~ U{20..120},
-element mix H/C/N/O 45/25/10/10 % plus 10 % uniform over the other elements
-up to Z = 54, total charges in {-1, 0, +1}, jittered 2.8 bohr grids;
-s6 = 1, s8 = 1.2, a1 = 0.40, a2 = 5.0 bohr, s9 = 1 (ATM on); cutoffs 50 / 25
-/ 25 bohr. One step = atom energies and the gradient of the total energy for
-the whole batch, inputs resident in HBM (fused kernel sequence).
+Default workload ``batch_1024_20_120``: 1024 synthetic molecules,
+n ~ U{20..120}, element mix H/C/N/O 45/25/10/10 % plus 10 % uniform over the
+other elements up to Z = 54, total charges in {-1, 0, +1}, jittered 2.8 bohr
+grids; s6 = 1, s8 = 1.2, a1 = 0.40, a2 = 5.0 bohr, s9 = 1 (ATM on); cutoffs
+50 / 25 / 25 bohr. One step = atom energies and the gradient of the total
+energy for the whole batch, inputs resident in HBM (fused kernel sequence).
 
 Prints one JSON line with the throughput (molecules/s), a roofline record for
 the dominant kernel (three-body term, timed with HIP events on its launch
 stream), the plain per-molecule GPU path and the FP64 CPU restatement timed
 on the host on bounded samples of the same workload.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other workloads (``--workload``), one JSON line each:
+  batch_256_8_200  256 molecules, n ~ U{8..200}, energy only, one launch;
+  cluster_5000     one 5000-atom cluster (H/C/N/O/S), energy + gradient;
+  serving_stream   10,000 requests (n ~ lognormal, median 40, 4..2000 atoms,
+                   70 % energy, 30 % energy + forces) served by the
+                   continuous-batching loop at 70 % of its measured capacity
+                   with Poisson arrivals: p50 / p99 latency.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 
 from __future__ import annotations
@@ -46,6 +54,7 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--requests", type=int, default=10000, help="serving_stream requests")
     return ap.parse_args()
 
 
@@ -59,7 +68,15 @@ def traffic_from_profiles(workload: str):
     return float(m.group(1)) if m else None
 
 
-def plain_gpu(mols, device, budget_s: float) -> dict:
+METRICS = {
+    "batch_1024_20_120": "mol/s E+forces (20-120 atoms) batched vs plain; p50/p99 serve latency",
+    "batch_256_8_200": "mol/s energy (8-200 atoms), one launch",
+    "cluster_5000": "5000-atom cluster energy + forces evaluations/s",
+}
+FORCES = {"batch_1024_20_120": True, "batch_256_8_200": False, "cluster_5000": True}
+
+
+def plain_gpu(mols, device, budget_s: float, workload: str = WORKLOAD) -> dict:
     import torch
 
     from tad_dftd4_amd import plain
@@ -84,11 +101,11 @@ def plain_gpu(mols, device, budget_s: float) -> dict:
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "molecules/s", "sample": f"first {done} molecules of "
-            f"{WORKLOAD}, one molecule per call, energy + autograd forces, dense torch ops on "
+            f"{workload}, one molecule per call, energy + autograd forces, dense torch ops on "
             "the GPU"}
 
 
-def cpu_baseline(mols, budget_s: float) -> dict:
+def cpu_baseline(mols, budget_s: float, workload: str = WORKLOAD, forces: bool = True) -> dict:
     import torch
 
     from oracle import dense
@@ -98,15 +115,71 @@ def cpu_baseline(mols, budget_s: float) -> dict:
     t0 = time.perf_counter()
     done = 0
     for m in mols:
-        dense.dftd4_with_grad(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge,
-                              par, cutoff=cut)
+        if forces:
+            dense.dftd4_with_grad(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge,
+                                  par, cutoff=cut)
+        else:
+            dense.dftd4(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge, par,
+                        cutoff=cut)
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
+    what = "energy + autograd forces" if forces else "energy"
     return {"value": done / dt, "unit": "molecules/s", "cores": torch.get_num_threads(),
-            "kind": "port", "sample": f"first {done} molecules of {WORKLOAD}, energy + autograd "
-            "forces, FP64 dense restatement on the host, one molecule per call"}
+            "kind": "port", "sample": f"first {done} molecules of {workload}, {what}, FP64 "
+            "dense restatement on the host, one molecule per call"}
+
+
+def serve_bench(args) -> dict:
+    """Continuous batching of a 10,000-request stream (see module docstring)."""
+    import numpy as np
+    import torch
+
+    from tad_dftd4_amd import synth
+    from tad_dftd4_amd.cutoff import Cutoff
+    from tad_dftd4_amd.serve import D4Server, Request, latency_stats
+
+    rng = np.random.default_rng(4)
+    sizes = synth.serving_sizes(rng, args.requests)
+    reqs = []
+    for k, n in enumerate(sizes):
+        m = synth.molecule(rng, int(n), synth.MIX_Z54, 2.8, (-2, -1, 0, 1, 2))
+        reqs.append(Request(k, m.numbers, m.positions, m.charge, forces=bool(rng.random() < 0.3)))
+    cut = Cutoff(disp2=CUT[0], disp3=CUT[1], cn_eeq=CUT[2])
+
+    def fresh():
+        return [Request(r.rid, r.numbers, r.positions, r.charge, r.forces) for r in reqs]
+
+    # warm-up on a slice, then closed-loop capacity: every request waiting at t = 0
+    D4Server(PARAM, cut).run_stream(fresh()[:500], np.zeros(500))
+    torch.cuda.synchronize()
+    cap = latency_stats(D4Server(PARAM, cut).run_stream(fresh(), np.zeros(len(reqs))))
+    rate = 0.7 * cap["molecules_per_s"]
+    arrivals = np.cumsum(np.random.default_rng(5).exponential(1.0 / rate, size=len(reqs)))
+    res = D4Server(PARAM, cut).run_stream(fresh(), arrivals)
+    st = latency_stats(res)
+    return {
+        "metric": "p50/p99 serve latency, 10k requests (4-2000 atoms, 70% E / 30% E+F), "
+                  "Poisson at 70% of capacity",
+        "value": round(st["p99_ms"], 3),
+        "unit": "ms (p99)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "p50_ms": round(st["p50_ms"], 3),
+        "p99_ms": round(st["p99_ms"], 3),
+        "mean_ms": round(st["mean_ms"], 3),
+        "molecules_per_s": round(st["molecules_per_s"], 1),
+        "capacity_molecules_per_s": round(cap["molecules_per_s"], 1),
+        "offered_rate_per_s": round(rate, 1),
+        "batches": st["batches"],
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "serving_stream", "requests": len(reqs),
+                   "atoms_min_max": [int(sizes.min()), int(sizes.max())],
+                   "atoms_median": float(np.median(sizes)), "cutoffs_bohr": list(CUT),
+                   "damping": PARAM, "parallelism": "single GPU"},
+    }
 
 
 def main() -> int:
@@ -126,6 +199,10 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload == "serving_stream":
+        print(json.dumps(serve_bench(args)), flush=True)
+        return 0
+    forces = FORCES[args.workload]
     mols = synth.workload(args.workload)
     numbers, positions, charge = synth.pack(mols, device=dev)
     B, N = numbers.shape
@@ -135,7 +212,11 @@ def main() -> int:
     grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
 
     def step():
-        runtime.energy_gradient(numbers, positions, charge, kp, ws=ws, energy=energy, grad=grad)
+        if forces:
+            runtime.energy_gradient(numbers, positions, charge, kp, ws=ws, energy=energy,
+                                    grad=grad)
+        else:
+            runtime.forward(numbers, positions, charge, kp, ws=ws)
 
     for _ in range(args.warmup):
         step()
@@ -156,14 +237,14 @@ def main() -> int:
     ms_step = ms_total / args.steps
     value = B * args.steps / (ms_total / 1e3)
 
-    work = metrics.workload_flops(mols, CUT[0], CUT[1], with_forces=True)
+    work = metrics.workload_flops(mols, CUT[0], CUT[1], with_forces=forces)
     atm_ms, atm_n = prof["atm"]
     atm_avg_s = atm_ms / max(atm_n, 1) / 1e3
     achieved = work["atm"] / atm_avg_s / 1e12
     kernel_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]}
 
     out = {
-        "metric": "mol/s E+forces (20-120 atoms) batched vs plain; p50/p99 serve latency",
+        "metric": METRICS[args.workload],
         "value": round(value, 1),
         "unit": "molecules/s",
         "n_gpus": 1,
@@ -181,13 +262,15 @@ def main() -> int:
             "atoms_min_max": [int(min(len(m.numbers) for m in mols)),
                               int(max(len(m.numbers) for m in mols))],
             "padded_atoms": N,
-            "outputs": "atom energies + gradient",
+            "outputs": "atom energies + gradient" if forces else "atom energies",
             "cutoffs_bohr": list(CUT),
             "damping": PARAM,
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": "k_atm (three-body, energy + gradient)",
+            "kernel": ("k_atm1 (three-body, each triple once" if N <= 256 else
+                       "k_atm (three-body, pair tiles") + (", energy + gradient)" if forces
+                                                           else ", energy)"),
             "bound": "mfma",
             "note": "FP64 VALU kernel (no MFMA); MI355X FP64 peak is 78.6 TFLOP/s for vector "
                     "and matrix alike",
@@ -206,11 +289,17 @@ def main() -> int:
         },
         "kernel_avg_ms": kernel_ms,
     }
-    if not args.no_baselines:
-        pl = plain_gpu(mols, dev, args.plain_seconds)
-        out["plain_gpu"] = pl
-        out["speedup_vs_plain"] = round(value / pl["value"], 1)
-        out["cpu_baseline"] = cpu_baseline(mols, args.cpu_seconds)
+    if args.workload == "cluster_5000":
+        out["unit"] = "evaluations/s"
+        out["cpu_baseline"] = {"value": None, "unit": "evaluations/s", "cores": None,
+                               "kind": "port", "sample": "not run: the dense restatement needs "
+                               "n^3 = 1.25e11-element tensors (1 TB each) for 5000 atoms"}
+    elif not args.no_baselines:
+        if forces:
+            pl = plain_gpu(mols, dev, args.plain_seconds, args.workload)
+            out["plain_gpu"] = pl
+            out["speedup_vs_plain"] = round(value / pl["value"], 1)
+        out["cpu_baseline"] = cpu_baseline(mols, args.cpu_seconds, args.workload, forces)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -39,6 +39,8 @@ extern "C" {
 #define D4_FLAG_S10 1      /* "s10" key present: evaluate the order-10 term */
 #define D4_FLAG_FORCE_LU 2    /* testing: solve every EEQ system by pivoted LU */
 #define D4_FLAG_FORCE_TILED 4 /* testing: use the large-system (tiled) EEQ path at any size */
+#define D4_FLAG_FORCE_SORT 8  /* testing: spatially sort the atoms of every molecule */
+#define D4_FLAG_FORCE_PAIR_TILES 16 /* testing: three-body term by pair tiles at any size */
 
 /* Element and reference-system tables, host memory, read once by
  * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference

@@ -24,6 +24,8 @@ D4_MAX_REF = 7
 D4_FLAG_S10 = 1
 D4_FLAG_FORCE_LU = 2
 D4_FLAG_FORCE_TILED = 4
+D4_FLAG_FORCE_SORT = 8
+D4_FLAG_FORCE_PAIR_TILES = 16
 
 SLOTS = ("prep", "cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain")
 

@@ -65,6 +65,8 @@ d4::Params to_params(const d4_params* p) {
   k.s10on = (p->flags & D4_FLAG_S10) ? 1 : 0;
   k.force_lu = (p->flags & D4_FLAG_FORCE_LU) ? 1 : 0;
   k.force_tiled = (p->flags & D4_FLAG_FORCE_TILED) ? 1 : 0;
+  k.force_sort = (p->flags & D4_FLAG_FORCE_SORT) ? 1 : 0;
+  k.force_pairtiles = (p->flags & D4_FLAG_FORCE_PAIR_TILES) ? 1 : 0;
   return k;
 }
 
@@ -91,11 +93,15 @@ int check_common(const d4_context* ctx, int64_t nbatch, int64_t natoms,
   return D4_OK;
 }
 
+// Molecules of at least this padded length run in spatial (Morton) order.
+constexpr int64_t kSortMinN = 256;
+
 struct Call {
   d4::Work w;
   d4::Inputs in;
   d4::Params p;
   hipStream_t s;
+  bool sorted;
 };
 
 Call make_call(void* stream, int64_t nbatch, int64_t natoms,
@@ -107,7 +113,36 @@ Call make_call(void* stream, int64_t nbatch, int64_t natoms,
   c.in = d4::Inputs{numbers, positions, charge, q_in, r4r2_in, nullptr};
   c.p = to_params(params);
   c.s = (hipStream_t)stream;
+  c.sorted = natoms >= kSortMinN || c.p.force_sort;
   return c;
+}
+
+// Sorted order: gather the inputs (sorting first unless `reuse` keeps the
+// order a previous call left in the workspace) and point the call at the
+// sorted copies.
+hipError_t enter_sorted(Call& c, bool reuse) {
+  if (!c.sorted) return hipSuccess;
+  hipError_t e = hipSuccess;
+  if (!reuse) e = d4::launch_sort(c.s, c.in, c.w);
+  if (e == hipSuccess) e = d4::launch_gather(c.s, c.in, c.w);
+  if (e != hipSuccess) return e;
+  c.in.numbers = c.w.snum;
+  c.in.positions = c.w.spos;
+  if (c.in.q_in) c.in.q_in = c.w.sq;
+  if (c.in.r4r2_in) c.in.r4r2_in = c.w.sr4;
+  if (c.in.gE) c.in.gE = c.w.sge;
+  return hipSuccess;
+}
+
+// Where a per-atom output is written by the kernels: the caller's buffer, or
+// the sorted-order staging buffer that leave_sorted() scatters from.
+double* out_atoms(const Call& c, double* user, double* staging) {
+  return c.sorted ? staging : user;
+}
+
+hipError_t scatter_atoms(const Call& c, const double* src, double* dst, int comps) {
+  if (!c.sorted || dst == nullptr) return hipSuccess;
+  return d4::launch_scatter_atoms(c.s, c.w, src, dst, comps);
 }
 
 hipEvent_t take_event(d4_context* ctx) {
@@ -141,6 +176,7 @@ hipError_t timed(d4_context* ctx, hipStream_t s, int slot, F&& fn) {
   D4_TRY(timed(ctx, c.s, slot, [&]() { return (expr); }), where)
 
 int run_front(d4_context* ctx, Call& c) {
+  D4_RUN(D4_SLOT_PREP, enter_sorted(c, false), "spatial order");
   D4_RUN(D4_SLOT_PREP, d4::launch_prep(c.s, c.in, c.w), "prep");
   D4_RUN(D4_SLOT_CN, d4::launch_cn(c.s, ctx->t, c.p, c.in, c.w), "coordination numbers");
   D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
@@ -259,11 +295,18 @@ int d4_forward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
                      params, workspace);
   if ((rc = run_front(ctx, c)) != D4_OK) return rc;
+  double* e_out = out_atoms(c, energy_out, c.w.sE);
   D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, false, nullptr), "pair pass");
   D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, true, false), "three-body term");
-  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.in, c.w, true, false, energy_out, nullptr),
+  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, true, false, e_out, nullptr),
          "energy rows");
-  if (q_out) D4_TRY(d4::launch_copy(c.s, c.w.q, q_out, nbatch * natoms), "copy charges");
+  D4_TRY(scatter_atoms(c, c.w.sE, energy_out, 1), "scatter energies");
+  if (q_out) {
+    if (c.sorted)
+      D4_TRY(scatter_atoms(c, c.w.q, q_out, 1), "scatter charges");
+    else
+      D4_TRY(d4::launch_copy(c.s, c.w.q, q_out, nbatch * natoms), "copy charges");
+  }
   return D4_OK;
 }
 
@@ -280,13 +323,16 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
                      params, workspace);
   c.in.gE = grad_energy;
-  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, grad_positions),
+  D4_RUN(D4_SLOT_PREP, enter_sorted(c, true), "spatial order");  // the forward's order
+  double* g_out = out_atoms(c, grad_positions, c.w.sG);
+  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, g_out),
          "pair pass (gradient)");
   D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, false, true), "three-body gradient");
-  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.in, c.w, false, true, nullptr, grad_positions),
+  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, false, true, nullptr, g_out),
          "three-body rows");
   D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
-  D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, grad_positions), "CN/EEQ chain");
+  D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, g_out), "CN/EEQ chain");
+  D4_TRY(scatter_atoms(c, c.w.sG, grad_positions, 3), "scatter gradient");
   return D4_OK;
 }
 
@@ -303,13 +349,17 @@ int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch, int64_t na
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
                      params, workspace);
   if ((rc = run_front(ctx, c)) != D4_OK) return rc;
-  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, grad_positions),
+  double* e_out = out_atoms(c, energy_out, c.w.sE);
+  double* g_out = out_atoms(c, grad_positions, c.w.sG);
+  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, g_out),
          "pair pass (gradient)");
   D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, true, true), "three-body term");
-  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.in, c.w, true, true, energy_out, grad_positions),
+  D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, true, true, e_out, g_out),
          "three-body rows");
   D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
-  D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, grad_positions), "CN/EEQ chain");
+  D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, g_out), "CN/EEQ chain");
+  D4_TRY(scatter_atoms(c, c.w.sE, energy_out, 1), "scatter energies");
+  D4_TRY(scatter_atoms(c, c.w.sG, grad_positions, 3), "scatter gradient");
   return D4_OK;
 }
 
@@ -323,6 +373,18 @@ int d4_properties(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, nullptr,
                      nullptr, params, workspace);
   if ((rc = run_front(ctx, c)) != D4_OK) return rc;
+  if (c.sorted) {
+    // C6 is staged in the (unused here) three-body gradient pair array
+    if (c6_out || alpha_out)
+      D4_TRY(d4::launch_properties(c.s, ctx->t, c.in, c.w, c6_out ? c.w.pg : nullptr,
+                                   alpha_out ? c.w.sE : nullptr),
+             "properties");
+    if (c6_out) D4_TRY(d4::launch_scatter_pairs(c.s, c.w, c.w.pg, c6_out), "scatter C6");
+    D4_TRY(scatter_atoms(c, c.w.sE, alpha_out, 1), "scatter polarisabilities");
+    D4_TRY(scatter_atoms(c, c.w.q, q_out, 1), "scatter charges");
+    D4_TRY(scatter_atoms(c, c.w.cn, cn_out, 1), "scatter coordination numbers");
+    return D4_OK;
+  }
   if (c6_out || alpha_out)
     D4_TRY(d4::launch_properties(c.s, ctx->t, c.in, c.w, c6_out, alpha_out),
            "properties");

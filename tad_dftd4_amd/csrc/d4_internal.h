@@ -47,6 +47,8 @@ struct Params {
   int s10on;
   int force_lu;
   int force_tiled;
+  int force_sort;
+  int force_pairtiles;
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).
@@ -78,8 +80,35 @@ struct Work {
   double* pg;       // [B*N*N] ATM dL/d r_ij^2
   double* ph;       // [B*N*N] ATM dL/d log sqrt|C6(q=0)_ij|
   double* pe;       // [B*N*N] ATM atom-energy partials (row atom)
+  // spatial order of large molecules (see launch_sort)
+  int32_t* perm;    // [B*N] sorted slot -> input index
+  int64_t* snum;    // [B*N] atomic numbers in sorted order
+  double* spos;     // [B*N*3] positions in sorted order
+  double* sq;       // [B*N] given charges in sorted order
+  double* sr4;      // [B*N] given r4r2 in sorted order
+  double* sge;      // [B*N] dL/dE in sorted order
+  double* sE;       // [B*N] per-atom outputs in sorted order
+  double* sG;       // [B*N*3] gradient in sorted order
+  double* gsph;     // [B*NG*4] bounding sphere (centre, radius) of each 16-atom group
+  double* apart;    // [B*S*5*N] per-split atom sums of the one-pass three-body kernel
   int64_t B, N;
 };
+
+constexpr int kGroup = 16;  // atoms per group (the three-body tile edge)
+__host__ __device__ inline int64_t n_groups(int64_t N) { return (N + kGroup - 1) / kGroup; }
+
+// One-pass three-body kernel (molecules of up to kAtm1MaxN atoms): a
+// molecule of n atoms is split over atm1_splits(n) work-groups, a number that
+// depends on n alone, so a molecule's result does not depend on its batch.
+constexpr int kAtm1MaxN = 256;
+constexpr int kAtm1Acc = 5;  // energy share, gradient (3), dL/dCN
+__host__ __device__ inline int atm1_splits(int n) {
+  const int nb = (n + kGroup - 1) / kGroup;
+  const int tiles = nb * (nb + 1) / 2;
+  const long long work = (long long)n * n * n;
+  const int s = (int)((work + 262143) / 262144);  // about one 64-atom molecule each
+  return tiles < 1 ? 1 : (s < 1 ? 1 : (s > tiles ? tiles : s));
+}
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -125,6 +154,18 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
   v.pg = (double*)take(8 * bn * size_t(N));
   v.ph = (double*)take(8 * bn * size_t(N));
   v.pe = (double*)take(8 * bn * size_t(N));
+  v.perm = (int32_t*)take(sizeof(int32_t) * bn);
+  v.snum = (int64_t*)take(8 * bn);
+  v.spos = (double*)take(24 * bn);
+  v.sq = (double*)take(8 * bn);
+  v.sr4 = (double*)take(8 * bn);
+  v.sge = (double*)take(8 * bn);
+  v.sE = (double*)take(8 * bn);
+  v.sG = (double*)take(24 * bn);
+  v.gsph = (double*)take(32 * size_t(B) * size_t(n_groups(N)));
+  v.apart = (double*)take(N <= kAtm1MaxN
+                              ? 8 * size_t(B) * size_t(atm1_splits((int)N)) * kAtm1Acc * size_t(N)
+                              : 0);
   if (w) *w = v;
   return off;
 }
@@ -152,7 +193,7 @@ hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         double* grad_out);
 hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
                       const Work& w, bool energy, bool grad);
-hipError_t launch_atm_rows(hipStream_t s, const Inputs& in, const Work& w,
+hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, const Work& w,
                            bool energy, bool grad, double* energy_out,
                            double* grad_out);
 hipError_t launch_adjoint(hipStream_t s, const DevTables& t, const Params& p,
@@ -163,5 +204,16 @@ hipError_t launch_properties(hipStream_t s, const DevTables& t,
                              const Inputs& in, const Work& w, double* c6_out,
                              double* alpha_out);
 hipError_t launch_copy(hipStream_t s, const double* src, double* dst, int64_t count);
+
+// Spatial order (d4_sort.hip). launch_sort computes w.perm (only when
+// `sort`); launch_gather fills the sorted copies of the inputs that are
+// non-null; the scatters write sorted-order results back in input order.
+hipError_t launch_sort(hipStream_t s, const Inputs& in, const Work& w);
+hipError_t launch_gather(hipStream_t s, const Inputs& in, const Work& w);
+hipError_t launch_scatter_atoms(hipStream_t s, const Work& w, const double* src,
+                                double* dst, int comps);
+hipError_t launch_scatter_pairs(hipStream_t s, const Work& w, const double* src,
+                                double* dst);
+hipError_t launch_groups(hipStream_t s, const Inputs& in, const Work& w);
 
 }  // namespace d4

@@ -330,7 +330,23 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
 // The cutoff screens r_ij and r_jk of every ordered (i, j, k) term and never
 // r_ik (threebody.py:153-160); per unordered triple this gives atom x the
 // share e/6 [yz]([xy] + [xz]).
+//
+// Screening by groups: a triple counts only if two of its three edges are
+// within the cutoff c3. With bounding spheres of the 16-atom groups (the
+// tile edges), a tile whose two groups are farther apart than 2 c3 has no
+// counted triple at all, and a third-atom group is streamed only if at least
+// two of the three group-to-group lower distance bounds are within c3. The
+// kept groups are listed in ascending order, so the third atoms are visited
+// in the same order with or without screening and skipped terms are exact
+// zeros: screening does not change a single bit of the result.
 // --------------------------------------------------------------------------
+
+constexpr int kMaxGroups = 2880;  // 46000 atoms / 16
+
+__device__ __forceinline__ double sphere_gap(const double* a, const double* b) {
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return sqrt(dx * dx + dy * dy + dz * dz) - a[3] - b[3];
+}
 
 template <bool ENERGY, bool GRAD>
 __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
@@ -339,6 +355,9 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
   __shared__ double s_p[kZB][2 * kTile];
   __shared__ double s_r[kZB][4];  // x, y, z, dL/dE (0 for padding)
   __shared__ int s_ok[kZB];
+  __shared__ int s_z[kZB];
+  __shared__ int s_glist[kMaxGroups];
+  __shared__ int s_cnt[kWaves];
 
   // spread consecutive logical tiles over one XCD: the hardware deals
   // blocks round-robin to the 8 XCDs, so block b runs on XCD b % 8
@@ -389,11 +408,44 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
   const double s9 = p.s9, alp = p.alp;
   double ex = 0.0, ey = 0.0, accg = 0.0, acch = 0.0;
 
-  for (int z0 = 0; z0 < n; z0 += kZB) {
+  // group screening (uniform over the work-group)
+  const double c3 = sqrt(p.cut3sq) * (1.0 + 1e-10) + 1e-10;  // margin for rounding in the bounds
+  const double* S = w.gsph + 4 * m * n_groups(N);
+  const double gxy = sphere_gap(S + 4 * bx, S + 4 * by);
+  if (gxy > 2.0 * c3) {  // no counted triple has this pair
+    if (valid) {
+      const int64_t ixy = (m * N + x) * N + y, iyx = (m * N + y) * N + x;
+      if (ENERGY) w.pe[ixy] = w.pe[iyx] = 0.0;
+      if (GRAD) w.pg[ixy] = w.pg[iyx] = w.ph[ixy] = w.ph[iyx] = 0.0;
+    }
+    return;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int total = 0;
+  for (int base = 0; base < nb; base += 256) {
+    const int gz = base + threadIdx.x;
+    bool keep = false;
+    if (gz < nb) {
+      const int near = (gxy <= c3) + (sphere_gap(S + 4 * bx, S + 4 * gz) <= c3) +
+                       (sphere_gap(S + 4 * by, S + 4 * gz) <= c3);
+      keep = near >= 2;
+    }
+    const unsigned long long mask = __ballot(keep);
+    if (lane == 0) s_cnt[wave] = __popcll(mask);
+    __syncthreads();
+    int off = total;
+    for (int k = 0; k < wave; ++k) off += s_cnt[k];
+    if (keep) s_glist[off + __popcll(mask & ((1ull << lane) - 1ull))] = gz;
+    for (int k = 0; k < kWaves; ++k) total += s_cnt[k];
+    __syncthreads();
+  }
+
+  for (int gi = 0; gi < total; gi += 2) {
     __syncthreads();
     for (int e = threadIdx.x; e < kZB * 4 * kTile; e += 256) {
       const int zl = e / (4 * kTile), c = e - zl * (4 * kTile);
-      const int z = z0 + zl;
+      const int gs = gi + (zl >> 4);
+      const int z = gs < total ? s_glist[gs] * kTile + (zl & (kTile - 1)) : n;
       const int half = c & (2 * kTile - 1);  // 0..31
       const int col = half < kTile ? bx * kTile + half : by * kTile + half - kTile;
       double v = 0.0;
@@ -404,8 +456,11 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
         s_p[zl][half] = v;
     }
     if (threadIdx.x < kZB) {
-      const int zl = threadIdx.x, z = z0 + zl;
+      const int zl = threadIdx.x;
+      const int gs = gi + (zl >> 4);
+      const int z = gs < total ? s_glist[gs] * kTile + (zl & (kTile - 1)) : n;
       const bool ok = z < n && Z[z] != 0;
+      s_z[zl] = z;
       s_ok[zl] = ok;
       s_r[zl][0] = ok ? R[3 * z] : 0.0;
       s_r[zl][1] = ok ? R[3 * z + 1] : 0.0;
@@ -414,9 +469,12 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
     }
     __syncthreads();
     if (!valid) continue;
-    const int zmax = min(kZB, n - z0);
+    // slots up to the last atom of the molecule (only the last group is partial)
+    const int z0 = s_glist[gi] * kTile;
+    const int zmax = gi + 1 < total ? kTile + min(kTile, n - s_glist[gi + 1] * kTile)
+                                    : min(kTile, n - z0);
     for (int zl = 0; zl < zmax; ++zl) {
-      const int z = z0 + zl;
+      const int z = s_z[zl];
       const bool vz = s_ok[zl] && z != x && z != y;
       const double rzx = s_r[zl][0], rzy = s_r[zl][1], rzz = s_r[zl][2];
       const double dxx = px - rzx, dxy = py - rzy, dxz = pz - rzz;
@@ -520,6 +578,292 @@ __global__ __launch_bounds__(256) void k_atm_rows(Inputs in, Work w,
       grad_out[3 * row + 1] += fy;
       grad_out[3 * row + 2] += fz;
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// ATM three-body term with each unordered triple evaluated once (molecules
+// of up to kAtm1MaxN atoms).
+//
+// Work-group (m, s) takes every S-th pair tile (bx <= by) of molecule m,
+// S = atm1_splits(n). Lane (tx, ty) owns the pair x = 16 bx + tx <
+// y = 16 by + ty and visits the third atoms z > y in chunks of 16. Inside a
+// chunk the 16 lanes of a row visit the chunk on a skewed schedule,
+// z = z0 + (k + tx) mod 16 at step k, and hand their running sums for z one
+// lane down after every step (a DPP row rotate): after 16 steps every lane
+// holds its row's complete sums for one z. Contributions of x and y stay in
+// the lane. Row, tile and split sums are added in a fixed order, so there
+// are no atomics and the result depends on the molecule alone.
+//
+// Per triple (threebody.py:54-163 with the partition of the asymmetric
+// screening): atom x gets e/6 [yz]([xy] + [xz]) (y, z alike), and the
+// gradient and CN chain use the weight W = sum over the three atoms of
+// dL/dE_atom times its share.
+// --------------------------------------------------------------------------
+
+// Lane i of each 16-lane row receives the value of lane i + 1 (mod 16).
+__device__ __forceinline__ double row_rotate_down(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x12F, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x12F, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// 1 / d by the hardware estimate and two Newton steps.
+__device__ __forceinline__ double fast_rcp(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+template <bool ENERGY, bool GRAD>
+__global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S,
+                                              int64_t nblocks) {
+  constexpr int NA = kAtm1Acc;
+  __shared__ double s_t[kTile][2 * kTile + 1];  // pt[z][x block | y block]
+  __shared__ double s_p[kTile][2 * kTile + 1];  // pp[z][...]
+  __shared__ double s_k[kTile][2 * kTile + 1];  // pk[z][...]  (d / d CN_z)
+  __shared__ double s_kx[kTile][kTile + 1];     // pk[x][z]    (d / d CN_x)
+  __shared__ double s_ky[kTile][kTile + 1];     // pk[y][z]
+  __shared__ double s_r[kTile][4];              // z position, dL/dE_z
+  __shared__ int s_zok[kTile];
+  __shared__ double s_red[kTile][kTile][NA];
+  __shared__ double s_acc[kAtm1MaxN][NA];
+
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t logical = (b % 8) * (G / 8) + b / 8;  // consecutive blocks on one XCD
+  if (logical >= nblocks) return;
+  const int64_t m = logical / S;
+  const int s = (int)(logical - m * S);
+  const int64_t N = w.N;
+  const int n = w.natom[m];
+  const int Sm = atm1_splits(n);  // this molecule's split count (S is the batch maximum)
+  if (s >= Sm) return;
+  const int nb = (n + kTile - 1) / kTile;
+  const int tiles = nb * (nb + 1) / 2;
+  const int tid = threadIdx.x, tx = tid & (kTile - 1), ty = tid >> 4;
+  const int64_t* Z = in.numbers + m * N;
+  const double* R = in.positions + m * N * 3;
+  const double* pt = w.pt + m * N * N;
+  const double* pp = w.pp + m * N * N;
+  const double* pk = w.pk + m * N * N;
+  const double c3sq = p.cut3sq, s9 = p.s9, alp = p.alp;
+  for (int e = tid; e < n * NA; e += 256) (&s_acc[0][0])[e] = 0.0;
+
+  for (int t = s; t < tiles; t += Sm) {
+    int by = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (by * (by + 1) / 2 > t) --by;
+    while ((by + 1) * (by + 2) / 2 <= t) ++by;
+    const int bx = t - by * (by + 1) / 2;
+    const int x = bx * kTile + tx, y = by * kTile + ty;
+    const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
+    double rx[3] = {0.0, 0.0, 0.0}, ry[3] = {0.0, 0.0, 0.0}, dxy[3] = {0.0, 0.0, 0.0};
+    double a = 1.0, txy = 0.0, pxy = 0.0, kxy = 0.0, kyx = 0.0, gx = 0.0, gy = 0.0;
+    double mxy = 0.0;
+    if (valid) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        rx[d] = R[3 * x + d];
+        ry[d] = R[3 * y + d];
+        dxy[d] = rx[d] - ry[d];
+      }
+      a = dxy[0] * dxy[0] + dxy[1] * dxy[1] + dxy[2] * dxy[2];
+      txy = pt[(int64_t)x * N + y];
+      pxy = pp[(int64_t)x * N + y];
+      mxy = a <= c3sq ? 1.0 : 0.0;
+      if (GRAD) {
+        kxy = pk[(int64_t)x * N + y];
+        kyx = pk[(int64_t)y * N + x];
+        gx = in.gE ? in.gE[m * N + x] : 1.0;
+        gy = in.gE ? in.gE[m * N + y] : 1.0;
+      }
+    }
+    double ex = 0.0, ey = 0.0, cx = 0.0, cy = 0.0;
+    double fx[3] = {0.0, 0.0, 0.0}, fy[3] = {0.0, 0.0, 0.0};
+
+    for (int z0 = by * kTile; z0 < n; z0 += kTile) {
+      __syncthreads();
+      for (int e = tid; e < kTile * 2 * kTile; e += 256) {
+        const int zl = e >> 5, c = e & 31;
+        const int z = z0 + zl;
+        const int col = c < kTile ? bx * kTile + c : by * kTile + c - kTile;
+        const bool ok = z < n && col < n;
+        const int64_t idx = (int64_t)z * N + col;
+        s_t[zl][c] = ok ? pt[idx] : 0.0;
+        s_p[zl][c] = ok ? pp[idx] : 0.0;
+        if (GRAD) s_k[zl][c] = ok ? pk[idx] : 0.0;
+      }
+      if (GRAD) {
+        for (int e = tid; e < 2 * kTile * kTile; e += 256) {
+          const int half = e >> 8, r = (e >> 4) & (kTile - 1), zl = e & (kTile - 1);
+          const int z = z0 + zl;
+          const int row = (half ? by : bx) * kTile + r;
+          const double v = (z < n && row < n) ? pk[(int64_t)row * N + z] : 0.0;
+          if (half)
+            s_ky[r][zl] = v;
+          else
+            s_kx[r][zl] = v;
+        }
+      }
+      if (tid < kTile) {
+        const int z = z0 + tid;
+        const bool ok = z < n && Z[z] != 0;
+        s_zok[tid] = ok;
+        s_r[tid][0] = ok ? R[3 * z] : 0.0;
+        s_r[tid][1] = ok ? R[3 * z + 1] : 0.0;
+        s_r[tid][2] = ok ? R[3 * z + 2] : 0.0;
+        s_r[tid][3] = (GRAD && ok) ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0;
+      }
+      __syncthreads();
+
+      double rE = 0.0, rD = 0.0, rF0 = 0.0, rF1 = 0.0, rF2 = 0.0;  // ring sums
+      for (int k = 0; k < kTile; ++k) {
+        const int zl = (k + tx) & (kTile - 1);
+        const int z = z0 + zl;
+        const bool vz = valid && z > y && s_zok[zl] != 0;
+        const double u0 = rx[0] - s_r[zl][0], u1 = rx[1] - s_r[zl][1], u2 = rx[2] - s_r[zl][2];
+        const double v0 = ry[0] - s_r[zl][0], v1 = ry[1] - s_r[zl][1], v2 = ry[2] - s_r[zl][2];
+        const double bb = vz ? u0 * u0 + u1 * u1 + u2 * u2 : 1.0;  // r_xz^2
+        const double cc = vz ? v0 * v0 + v1 * v1 + v2 * v2 : 1.0;  // r_yz^2
+        const double T = txy * s_t[zl][tx] * s_t[zl][kTile + ty];
+        const double P = vz ? pxy * s_p[zl][tx] * s_p[zl][kTile + ty] : 0.0;
+        const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
+        const double sabc = X * Y * W3;
+        const double ab = a * bb;
+        const double abc = ab * cc;
+        const double opT = fma(6.0, T, 1.0);
+        const double q0 = fast_rcp(abc * opT);
+        const double f = abc * q0;      // 1 / (1 + 6 T)
+        const double iabc = opT * q0;   // 1 / (a b c)
+        const double g = fma(0.375 * sabc, iabc, 1.0);
+        const double K = s9 * P * f;
+        const double e = K * g;
+        const double mxz = bb <= c3sq ? 1.0 : 0.0, myz = cc <= c3sq ? 1.0 : 0.0;
+        const double sx = myz * (mxy + mxz), sy = mxz * (mxy + myz), sz = mxy * (mxz + myz);
+        if (ENERGY) {
+          ex = fma(sx, e, ex);
+          ey = fma(sy, e, ey);
+          rE = fma(sz, e, rE);
+        }
+        if (GRAD) {
+          const double W6 = gx * sx + gy * sy + s_r[zl][3] * sz;
+          const double We = W6 * e;
+          const double G2 = 0.375 * iabc;
+          const double H = g * (alp * T * f - 1.5) - G2 * sabc;
+          const double ia = bb * cc * iabc, ib = a * cc * iabc, ic = ab * iabc;
+          const double XY = X * Y, XW = X * W3, YW = Y * W3;
+          const double KW = 2.0 * K * W6;
+          const double ca = KW * fma(H, ia, G2 * (YW + XW - XY));
+          const double cb = KW * fma(H, ib, G2 * (YW - XW + XY));
+          const double cg = KW * fma(H, ic, G2 * (XW + XY - YW));
+          fx[0] += ca * dxy[0] + cb * u0;
+          fx[1] += ca * dxy[1] + cb * u1;
+          fx[2] += ca * dxy[2] + cb * u2;
+          fy[0] += cg * v0 - ca * dxy[0];
+          fy[1] += cg * v1 - ca * dxy[1];
+          fy[2] += cg * v2 - ca * dxy[2];
+          rF0 -= cb * u0 + cg * v0;
+          rF1 -= cb * u1 + cg * v1;
+          rF2 -= cb * u2 + cg * v2;
+          cx = fma(We, kxy + s_kx[tx][zl], cx);
+          cy = fma(We, kyx + s_ky[ty][zl], cy);
+          rD = fma(We, s_k[zl][tx] + s_k[zl][kTile + ty], rD);
+        }
+        if (ENERGY) rE = row_rotate_down(rE);
+        if (GRAD) {
+          rF0 = row_rotate_down(rF0);
+          rF1 = row_rotate_down(rF1);
+          rF2 = row_rotate_down(rF2);
+          rD = row_rotate_down(rD);
+        }
+      }
+      // lane tx of every row now holds that row's sums for z0 + tx
+      s_red[ty][tx][0] = rE;
+      s_red[ty][tx][1] = rF0;
+      s_red[ty][tx][2] = rF1;
+      s_red[ty][tx][3] = rF2;
+      s_red[ty][tx][4] = rD;
+      __syncthreads();
+      if (tid < kTile * NA) {
+        const int zl = tid & (kTile - 1), c = tid >> 4;
+        if ((c == 0 ? ENERGY : GRAD) && z0 + zl < n) {
+          double sum = 0.0;
+          for (int r = 0; r < kTile; ++r) sum += s_red[r][zl][c];
+          s_acc[z0 + zl][c] += sum;
+        }
+      }
+    }
+    // tile sums of the x atoms (over the rows) and the y atoms (over the columns)
+    __syncthreads();
+    s_red[ty][tx][0] = ex;
+    s_red[ty][tx][1] = fx[0];
+    s_red[ty][tx][2] = fx[1];
+    s_red[ty][tx][3] = fx[2];
+    s_red[ty][tx][4] = cx;
+    __syncthreads();
+    if (tid < kTile * NA) {
+      const int col = tid & (kTile - 1), c = tid >> 4;
+      if ((c == 0 ? ENERGY : GRAD) && bx * kTile + col < n) {
+        double sum = 0.0;
+        for (int r = 0; r < kTile; ++r) sum += s_red[r][col][c];
+        s_acc[bx * kTile + col][c] += sum;
+      }
+    }
+    __syncthreads();
+    s_red[ty][tx][0] = ey;
+    s_red[ty][tx][1] = fy[0];
+    s_red[ty][tx][2] = fy[1];
+    s_red[ty][tx][3] = fy[2];
+    s_red[ty][tx][4] = cy;
+    __syncthreads();
+    if (tid < kTile * NA) {
+      const int row = tid & (kTile - 1), c = tid >> 4;
+      if ((c == 0 ? ENERGY : GRAD) && by * kTile + row < n) {
+        double sum = 0.0;
+        for (int q = 0; q < kTile; ++q) sum += s_red[row][q][c];
+        s_acc[by * kTile + row][c] += sum;
+      }
+    }
+  }
+  __syncthreads();
+  double* part = w.apart + (m * (int64_t)S + s) * NA * N;
+  for (int e = tid; e < NA * n; e += 256) {
+    const int c = e / n, i = e - c * n;
+    part[c * N + i] = s_acc[i][c];
+  }
+}
+
+// Atom sums of the one-pass three-body kernel over its splits: energies
+// (with the two-body part), gradient and dL/dCN.
+template <bool ENERGY, bool GRAD>
+__global__ __launch_bounds__(256) void k_atm1_rows(Inputs in, Work w, double* energy_out,
+                                                   double* grad_out) {
+  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (row >= w.B * w.N) return;
+  const int64_t N = w.N;
+  const int64_t m = row / N;
+  const int i = (int)(row - m * N);
+  const int n = w.natom[m];
+  const int zi = i < n ? (int)in.numbers[row] : 0;
+  if (zi == 0) {
+    if (ENERGY) energy_out[row] = 0.0;
+    return;
+  }
+  const int S = atm1_splits(n);
+  const int64_t stride = (int64_t)atm1_splits((int)N) * kAtm1Acc * N;
+  const double* part = w.apart + m * stride + i;
+  double acc[kAtm1Acc] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < S; ++s)
+#pragma unroll
+    for (int c = 0; c < kAtm1Acc; ++c) acc[c] += part[(s * kAtm1Acc + c) * N];
+  const double sixth = 1.0 / 6.0;
+  if (ENERGY) energy_out[row] = w.e2[row] + acc[0] * sixth;
+  if (GRAD) {
+    grad_out[3 * row] += acc[1] * sixth;
+    grad_out[3 * row + 1] += acc[2] * sixth;
+    grad_out[3 * row + 2] += acc[3] * sixth;
+    w.dldcn[row] += acc[4] * sixth;
   }
 }
 
@@ -671,13 +1015,33 @@ hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
   return hipGetLastError();
 }
 
+hipError_t launch_atm1(hipStream_t s, const Params& p, const Inputs& in, const Work& w,
+                       bool energy, bool grad) {
+  const int S = atm1_splits((int)w.N);
+  const int64_t nblocks = w.B * S;
+  const int64_t grid = (nblocks + 7) / 8 * 8;  // multiple of 8 for the XCD remap
+  if (energy && grad)
+    hipLaunchKernelGGL((k_atm1<true, true>), dim3((unsigned)grid), dim3(256), 0, s, p, in, w,
+                       S, nblocks);
+  else if (energy)
+    hipLaunchKernelGGL((k_atm1<true, false>), dim3((unsigned)grid), dim3(256), 0, s, p, in, w,
+                       S, nblocks);
+  else if (grad)
+    hipLaunchKernelGGL((k_atm1<false, true>), dim3((unsigned)grid), dim3(256), 0, s, p, in, w,
+                       S, nblocks);
+  return hipGetLastError();
+}
+
 hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
                       const Work& w, bool energy, bool grad) {
+  if (w.N <= kAtm1MaxN && !p.force_pairtiles) return launch_atm1(s, p, in, w, energy, grad);
   const int nb = (int)((w.N + kTile - 1) / kTile);
   const int tiles = nb * (nb + 1) / 2;
   const int64_t ntiles = w.B * tiles;
   const int64_t grid = (ntiles + 7) / 8 * 8;  // multiple of 8 for the XCD remap
   if (grid == 0) return hipSuccess;
+  hipError_t e = launch_groups(s, in, w);
+  if (e != hipSuccess) return e;
   if (energy && grad)
     hipLaunchKernelGGL((k_atm<true, true>), dim3((unsigned)grid), dim3(256), 0, s, p,
                        in, w, tiles, ntiles);
@@ -690,9 +1054,22 @@ hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
   return hipGetLastError();
 }
 
-hipError_t launch_atm_rows(hipStream_t s, const Inputs& in, const Work& w,
+hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, const Work& w,
                            bool energy, bool grad, double* energy_out,
                            double* grad_out) {
+  if (w.N <= kAtm1MaxN && !p.force_pairtiles) {
+    const unsigned grid = (unsigned)((w.B * w.N + 255) / 256);
+    if (energy && grad)
+      hipLaunchKernelGGL((k_atm1_rows<true, true>), dim3(grid), dim3(256), 0, s, in, w,
+                         energy_out, grad_out);
+    else if (energy)
+      hipLaunchKernelGGL((k_atm1_rows<true, false>), dim3(grid), dim3(256), 0, s, in, w,
+                         energy_out, grad_out);
+    else if (grad)
+      hipLaunchKernelGGL((k_atm1_rows<false, true>), dim3(grid), dim3(256), 0, s, in, w,
+                         energy_out, grad_out);
+    return hipGetLastError();
+  }
   if (energy && grad)
     hipLaunchKernelGGL((k_atm_rows<true, true>), dim3(rows_grid(w)), dim3(256), 0, s,
                        in, w, energy_out, grad_out);

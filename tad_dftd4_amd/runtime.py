@@ -95,12 +95,16 @@ class KernelParams:
     s10_on: bool = False
     force_lu: bool = False
     force_tiled: bool = False
+    force_sort: bool = False
+    force_pair_tiles: bool = False
 
     def c(self) -> _lib.Params:
         flags = (
             (_lib.D4_FLAG_S10 if self.s10_on else 0)
             | (_lib.D4_FLAG_FORCE_LU if self.force_lu else 0)
             | (_lib.D4_FLAG_FORCE_TILED if self.force_tiled else 0)
+            | (_lib.D4_FLAG_FORCE_SORT if self.force_sort else 0)
+            | (_lib.D4_FLAG_FORCE_PAIR_TILES if self.force_pair_tiles else 0)
         )
         return _lib.Params(
             self.s6, self.s8, self.s10, self.s9, self.a1, self.a2, self.alp,

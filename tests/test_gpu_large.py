@@ -91,3 +91,112 @@ def test_tiled_eeq_vs_oracle_n260():
     assert st[0] == 0
     assert (e[0] - eo).abs().max().item() < E_TOL
     assert (g[0] - go).abs().max().item() < G_TOL
+
+
+def _kp(cut=None, **flags):
+    kp = kernel_params(PARAM, cut or _cut(), None)
+    for k, v in flags.items():
+        setattr(kp, k, v)
+    return kp
+
+
+def test_spatial_order_matches_input_order():
+    """Sorted (Morton) order, forced on small molecules: energies, gradients
+    (fused and split forward/backward) and properties come back in input
+    order and agree with the unsorted path."""
+    _say("forced spatial sort, 3 molecules incl. inner padding")
+    mols = [_mol(100, 50), _mol(101, 23), _mol(102, 41)]
+    numbers, positions, charge = synth.pack(mols, device="cuda")
+    numbers[2, 5] = 0  # inner padding atom
+    positions[2, 5] = 0.0
+    ref = _kp()
+    srt = _kp(force_sort=True)
+    e1, g1, _ = runtime.energy_gradient(numbers, positions, charge, ref)
+    e2, g2, _ = runtime.energy_gradient(numbers, positions, charge, srt)
+    assert (e1 - e2).abs().max().item() < 1e-12
+    assert (g1 - g2).abs().max().item() < 1e-11
+    e3, ws = runtime.forward(numbers, positions, charge, srt)
+    gE = torch.rand_like(e3)
+    g3 = runtime.backward(numbers, positions, charge, srt, ws, grad_energy=gE)
+    ws1 = runtime.forward(numbers, positions, charge, ref)[1]
+    g4 = runtime.backward(numbers, positions, charge, ref, ws1, grad_energy=gE)
+    assert (e1 - e3).abs().max().item() < 1e-12
+    assert (g3 - g4).abs().max().item() < 1e-11
+    for a, b in zip(runtime.properties(numbers, positions, charge, ref),
+                    runtime.properties(numbers, positions, charge, srt)):
+        assert (a - b).abs().max().item() < 1e-11
+    assert float(e2[2, 5]) == 0.0 and float(g2[2, 5].abs().sum()) == 0.0
+
+
+def test_screened_cluster_n300_vs_oracle():
+    """n = 300 takes the large-system path by itself (spatial order, tiled
+    EEQ); short cutoffs make the group screening of the three-body term skip
+    tiles and third-atom groups."""
+    _say("n=300 cluster, cutoffs 20/8/10, vs oracle")
+    mol = _mol(110, 300, mix=synth.MIX_HCNOS, spacing=2.6, charges=(0,))
+    cut = {"disp2": 20.0, "disp3": 8.0, "cn_eeq": 10.0}
+    numbers = torch.tensor(mol.numbers)
+    positions = torch.tensor(mol.positions)
+    eo, go = dense.dftd4_with_grad(numbers, positions, 0.0,
+                                   {k: torch.tensor(v, dtype=F64) for k, v in PARAM.items()},
+                                   cutoff=cut)
+    kp = _kp(d4.Cutoff(disp2=cut["disp2"], disp3=cut["disp3"], cn=25.0, cn_eeq=cut["cn_eeq"]))
+    nt, pt, ct = synth.pack([mol], device="cuda")
+    e, g, ws = runtime.energy_gradient(nt, pt, ct, kp)
+    assert runtime.molecule_status(ws, 1)[0] == 0
+    assert (e[0].cpu() - eo).abs().max().item() < E_TOL
+    assert (g[0].cpu() - go).abs().max().item() < G_TOL
+
+
+def test_cluster_5000_invariants():
+    """The 5000-atom cluster at its real size: too large for the dense oracle
+    (n^3 tensors), so size-independent properties are checked: translation
+    invariance, zero net force, and central finite differences of the total
+    energy on three coordinates."""
+    _say("cluster_5000: translation, net force, finite differences")
+    mol = synth.workload("cluster_5000")[0]
+    kp = _kp()
+    nt, pt, ct = synth.pack([mol], device="cuda")
+    e, g, ws = runtime.energy_gradient(nt, pt, ct, kp)
+    assert runtime.molecule_status(ws, 1)[0] == 0
+    etot = e.sum().item()
+    shift = torch.tensor([3.7, -1.3, 0.25], dtype=F64, device="cuda")
+    e2, g2, _ = runtime.energy_gradient(nt, pt + shift, ct, kp)
+    assert abs(e2.sum().item() - etot) < 1e-11 * abs(etot)
+    assert (g2 - g).abs().max().item() < 1e-11
+    assert g.sum(dim=1).abs().max().item() < 1e-9
+    h = 1e-4
+    for atom, d in ((17, 0), (2500, 1), (4999, 2)):
+        p1 = pt.clone()
+        p1[0, atom, d] += h
+        p2 = pt.clone()
+        p2[0, atom, d] -= h
+        ep = runtime.forward(nt, p1, ct, kp)[0].sum().item()
+        em = runtime.forward(nt, p2, ct, kp)[0].sum().item()
+        fd = (ep - em) / (2 * h)
+        assert abs(fd - g[0, atom, d].item()) < 1e-8, (atom, d, fd, g[0, atom, d].item())
+
+
+def test_one_pass_three_body_matches_pair_tiles():
+    """The one-pass three-body kernel (each triple once, molecules up to 256
+    atoms) against the pair-tile kernel (each triple from all three pairs):
+    energies, fused gradient, and a weighted split backward; sizes cross the
+    16-atom tile edges, short cutoffs exercise the screening masks."""
+    _say("one-pass three-body vs pair tiles, n = 5, 16, 17, 40, 77, 130")
+    mols = [_mol(120 + k, n) for k, n in enumerate((5, 16, 17, 40, 77, 130))]
+    numbers, positions, charge = synth.pack(mols, device="cuda")
+    for cut in (_cut(), d4.Cutoff(disp2=30.0, disp3=7.0, cn=25.0, cn_eeq=12.0)):
+        one = _kp(cut)
+        ref = _kp(cut, force_pair_tiles=True)
+        e1, g1, _ = runtime.energy_gradient(numbers, positions, charge, one)
+        e2, g2, _ = runtime.energy_gradient(numbers, positions, charge, ref)
+        assert (e1 - e2).abs().max().item() < 1e-13
+        assert (g1 - g2).abs().max().item() < 1e-12
+        eo, ws1 = runtime.forward(numbers, positions, charge, one)
+        ep, ws2 = runtime.forward(numbers, positions, charge, ref)
+        assert (eo - e1).abs().max().item() < 1e-14
+        assert (eo - ep).abs().max().item() < 1e-13
+        gE = torch.rand_like(eo)
+        ga = runtime.backward(numbers, positions, charge, one, ws1, grad_energy=gE)
+        gb = runtime.backward(numbers, positions, charge, ref, ws2, grad_energy=gE)
+        assert (ga - gb).abs().max().item() < 1e-12
