@@ -603,9 +603,9 @@ __global__ __launch_bounds__(256) void k_atm_rows(Inputs in, Work w,
 
 // Lane i of each 16-lane row receives the value of lane i + 1 (mod 16).
 __device__ __forceinline__ double row_rotate_down(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x12F, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x12F, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x12F, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, 0x12F, 0xF, 0xF, false));
 }
 
 // 1 / d by the hardware estimate and two Newton steps.
@@ -617,17 +617,21 @@ __device__ __forceinline__ double fast_rcp(double d) {
   return fma(r, e, r);
 }
 
+// Pair data of (z, c) staged for the one-pass kernel, c an atom of the tile.
+struct alignas(32) ZPair {
+  double t;    // (R0 / r)^(alp / 3)
+  double p;    // sqrt|C6(q = 0)| / r^3
+  double kz;   // d log sqrt|C6(0)| / d CN_z
+  double kc;   // d log sqrt|C6(0)| / d CN_c
+};
+
 template <bool ENERGY, bool GRAD>
 __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S,
                                               int64_t nblocks) {
   constexpr int NA = kAtm1Acc;
-  __shared__ double s_t[kTile][2 * kTile + 1];  // pt[z][x block | y block]
-  __shared__ double s_p[kTile][2 * kTile + 1];  // pp[z][...]
-  __shared__ double s_k[kTile][2 * kTile + 1];  // pk[z][...]  (d / d CN_z)
-  __shared__ double s_kx[kTile][kTile + 1];     // pk[x][z]    (d / d CN_x)
-  __shared__ double s_ky[kTile][kTile + 1];     // pk[y][z]
-  __shared__ double s_r[kTile][4];              // z position, dL/dE_z
-  __shared__ int s_zok[kTile];
+  constexpr double kFar = 1.0e6;  // position of padding third atoms (bohr)
+  __shared__ ZPair s_zc[kTile][2 * kTile + 1];  // [z][x block | y block]
+  __shared__ double4 s_r[kTile];                // z position, dL/dE_z
   __shared__ double s_red[kTile][kTile][NA];
   __shared__ double s_acc[kAtm1MaxN][NA];
 
@@ -658,17 +662,19 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
     const int bx = t - by * (by + 1) / 2;
     const int x = bx * kTile + tx, y = by * kTile + ty;
     const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
-    double rx[3] = {0.0, 0.0, 0.0}, ry[3] = {0.0, 0.0, 0.0}, dxy[3] = {0.0, 0.0, 0.0};
-    double a = 1.0, txy = 0.0, pxy = 0.0, kxy = 0.0, kyx = 0.0, gx = 0.0, gy = 0.0;
+    double rx0 = 0.0, rx1 = 0.0, rx2 = 0.0, ry0 = 0.0, ry1 = 0.0, ry2 = 0.0;
+    double a = 1.0, ia = 1.0, txy = 0.0, pxy = 0.0, kxy = 0.0, kyx = 0.0, gx = 0.0, gy = 0.0;
     double mxy = 0.0;
     if (valid) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        rx[d] = R[3 * x + d];
-        ry[d] = R[3 * y + d];
-        dxy[d] = rx[d] - ry[d];
-      }
-      a = dxy[0] * dxy[0] + dxy[1] * dxy[1] + dxy[2] * dxy[2];
+      rx0 = R[3 * x];
+      rx1 = R[3 * x + 1];
+      rx2 = R[3 * x + 2];
+      ry0 = R[3 * y];
+      ry1 = R[3 * y + 1];
+      ry2 = R[3 * y + 2];
+      const double d0 = rx0 - ry0, d1 = rx1 - ry1, d2 = rx2 - ry2;
+      a = d0 * d0 + d1 * d1 + d2 * d2;
+      ia = 1.0 / a;
       txy = pt[(int64_t)x * N + y];
       pxy = pp[(int64_t)x * N + y];
       mxy = a <= c3sq ? 1.0 : 0.0;
@@ -679,8 +685,9 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
         gy = in.gE ? in.gE[m * N + y] : 1.0;
       }
     }
+    const double dxy0 = rx0 - ry0, dxy1 = rx1 - ry1, dxy2 = rx2 - ry2;
     double ex = 0.0, ey = 0.0, cx = 0.0, cy = 0.0;
-    double fx[3] = {0.0, 0.0, 0.0}, fy[3] = {0.0, 0.0, 0.0};
+    double fx0 = 0.0, fx1 = 0.0, fx2 = 0.0, fy0 = 0.0, fy1 = 0.0, fy2 = 0.0;
 
     for (int z0 = by * kTile; z0 < n; z0 += kTile) {
       __syncthreads();
@@ -688,54 +695,51 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
         const int zl = e >> 5, c = e & 31;
         const int z = z0 + zl;
         const int col = c < kTile ? bx * kTile + c : by * kTile + c - kTile;
-        const bool ok = z < n && col < n;
-        const int64_t idx = (int64_t)z * N + col;
-        s_t[zl][c] = ok ? pt[idx] : 0.0;
-        s_p[zl][c] = ok ? pp[idx] : 0.0;
-        if (GRAD) s_k[zl][c] = ok ? pk[idx] : 0.0;
-      }
-      if (GRAD) {
-        for (int e = tid; e < 2 * kTile * kTile; e += 256) {
-          const int half = e >> 8, r = (e >> 4) & (kTile - 1), zl = e & (kTile - 1);
-          const int z = z0 + zl;
-          const int row = (half ? by : bx) * kTile + r;
-          const double v = (z < n && row < n) ? pk[(int64_t)row * N + z] : 0.0;
-          if (half)
-            s_ky[r][zl] = v;
-          else
-            s_kx[r][zl] = v;
+        ZPair v = {0.0, 0.0, 0.0, 0.0};
+        if (z < n && col < n) {
+          const int64_t idx = (int64_t)z * N + col;
+          v.t = pt[idx];
+          v.p = pp[idx];
+          if (GRAD) {
+            v.kz = pk[idx];
+            v.kc = pk[(int64_t)col * N + z];
+          }
         }
+        s_zc[zl][c] = v;
       }
       if (tid < kTile) {
         const int z = z0 + tid;
         const bool ok = z < n && Z[z] != 0;
-        s_zok[tid] = ok;
-        s_r[tid][0] = ok ? R[3 * z] : 0.0;
-        s_r[tid][1] = ok ? R[3 * z + 1] : 0.0;
-        s_r[tid][2] = ok ? R[3 * z + 2] : 0.0;
-        s_r[tid][3] = (GRAD && ok) ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0;
+        s_r[tid] = ok ? make_double4(R[3 * z], R[3 * z + 1], R[3 * z + 2],
+                                     GRAD ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0)
+                      : make_double4(kFar, kFar, kFar, 0.0);
       }
       __syncthreads();
 
       double rE = 0.0, rD = 0.0, rF0 = 0.0, rF1 = 0.0, rF2 = 0.0;  // ring sums
+#pragma unroll 2
       for (int k = 0; k < kTile; ++k) {
         const int zl = (k + tx) & (kTile - 1);
-        const int z = z0 + zl;
-        const bool vz = valid && z > y && s_zok[zl] != 0;
-        const double u0 = rx[0] - s_r[zl][0], u1 = rx[1] - s_r[zl][1], u2 = rx[2] - s_r[zl][2];
-        const double v0 = ry[0] - s_r[zl][0], v1 = ry[1] - s_r[zl][1], v2 = ry[2] - s_r[zl][2];
-        const double bb = vz ? u0 * u0 + u1 * u1 + u2 * u2 : 1.0;  // r_xz^2
-        const double cc = vz ? v0 * v0 + v1 * v1 + v2 * v2 : 1.0;  // r_yz^2
-        const double T = txy * s_t[zl][tx] * s_t[zl][kTile + ty];
-        const double P = vz ? pxy * s_p[zl][tx] * s_p[zl][kTile + ty] : 0.0;
+        const double4 rz = s_r[zl];
+        const ZPair zx = s_zc[zl][tx], zy = s_zc[zl][kTile + ty];
+        // third atoms up to y, outside the molecule or padding carry no weight
+        const double on = (valid && z0 + zl > y) ? 1.0 : 0.0;
+        const double u0 = rx0 - rz.x, u1 = rx1 - rz.y, u2 = rx2 - rz.z;
+        const double v0 = ry0 - rz.x, v1 = ry1 - rz.y, v2 = ry2 - rz.z;
+        const double b2 = fma(u2, u2, fma(u1, u1, u0 * u0));
+        const double c2 = fma(v2, v2, fma(v1, v1, v0 * v0));
+        const double bb = on != 0.0 ? b2 : 1.0;  // r_xz^2
+        const double cc = on != 0.0 ? c2 : 1.0;  // r_yz^2
+        const double T = txy * zx.t * zy.t;
+        const double P = on * pxy * zx.p * zy.p;
         const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
         const double sabc = X * Y * W3;
-        const double ab = a * bb;
-        const double abc = ab * cc;
+        const double bc = bb * cc;
+        const double abc = a * bc;
         const double opT = fma(6.0, T, 1.0);
         const double q0 = fast_rcp(abc * opT);
-        const double f = abc * q0;      // 1 / (1 + 6 T)
-        const double iabc = opT * q0;   // 1 / (a b c)
+        const double f = abc * q0;     // 1 / (1 + 6 T)
+        const double iabc = opT * q0;  // 1 / (a b c)
         const double g = fma(0.375 * sabc, iabc, 1.0);
         const double K = s9 * P * f;
         const double e = K * g;
@@ -747,28 +751,28 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
           rE = fma(sz, e, rE);
         }
         if (GRAD) {
-          const double W6 = gx * sx + gy * sy + s_r[zl][3] * sz;
+          const double W6 = fma(gx, sx, fma(gy, sy, rz.w * sz));
           const double We = W6 * e;
           const double G2 = 0.375 * iabc;
-          const double H = g * (alp * T * f - 1.5) - G2 * sabc;
-          const double ia = bb * cc * iabc, ib = a * cc * iabc, ic = ab * iabc;
+          const double H = fma(g, fma(alp * T, f, -1.5), -G2 * sabc);
+          const double ibc = a * iabc;  // 1 / (b c)
           const double XY = X * Y, XW = X * W3, YW = Y * W3;
           const double KW = 2.0 * K * W6;
           const double ca = KW * fma(H, ia, G2 * (YW + XW - XY));
-          const double cb = KW * fma(H, ib, G2 * (YW - XW + XY));
-          const double cg = KW * fma(H, ic, G2 * (XW + XY - YW));
-          fx[0] += ca * dxy[0] + cb * u0;
-          fx[1] += ca * dxy[1] + cb * u1;
-          fx[2] += ca * dxy[2] + cb * u2;
-          fy[0] += cg * v0 - ca * dxy[0];
-          fy[1] += cg * v1 - ca * dxy[1];
-          fy[2] += cg * v2 - ca * dxy[2];
-          rF0 -= cb * u0 + cg * v0;
-          rF1 -= cb * u1 + cg * v1;
-          rF2 -= cb * u2 + cg * v2;
-          cx = fma(We, kxy + s_kx[tx][zl], cx);
-          cy = fma(We, kyx + s_ky[ty][zl], cy);
-          rD = fma(We, s_k[zl][tx] + s_k[zl][kTile + ty], rD);
+          const double cb = KW * fma(H, cc * ibc, G2 * (YW - XW + XY));
+          const double cg = KW * fma(H, bb * ibc, G2 * (XW + XY - YW));
+          fx0 = fma(ca, dxy0, fma(cb, u0, fx0));
+          fx1 = fma(ca, dxy1, fma(cb, u1, fx1));
+          fx2 = fma(ca, dxy2, fma(cb, u2, fx2));
+          fy0 = fma(cg, v0, fma(-ca, dxy0, fy0));
+          fy1 = fma(cg, v1, fma(-ca, dxy1, fy1));
+          fy2 = fma(cg, v2, fma(-ca, dxy2, fy2));
+          rF0 = fma(-cb, u0, fma(-cg, v0, rF0));
+          rF1 = fma(-cb, u1, fma(-cg, v1, rF1));
+          rF2 = fma(-cb, u2, fma(-cg, v2, rF2));
+          cx = fma(We, kxy + zx.kc, cx);
+          cy = fma(We, kyx + zy.kc, cy);
+          rD = fma(We, zx.kz + zy.kz, rD);
         }
         if (ENERGY) rE = row_rotate_down(rE);
         if (GRAD) {
@@ -797,9 +801,9 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
     // tile sums of the x atoms (over the rows) and the y atoms (over the columns)
     __syncthreads();
     s_red[ty][tx][0] = ex;
-    s_red[ty][tx][1] = fx[0];
-    s_red[ty][tx][2] = fx[1];
-    s_red[ty][tx][3] = fx[2];
+    s_red[ty][tx][1] = fx0;
+    s_red[ty][tx][2] = fx1;
+    s_red[ty][tx][3] = fx2;
     s_red[ty][tx][4] = cx;
     __syncthreads();
     if (tid < kTile * NA) {
@@ -812,9 +816,9 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
     }
     __syncthreads();
     s_red[ty][tx][0] = ey;
-    s_red[ty][tx][1] = fy[0];
-    s_red[ty][tx][2] = fy[1];
-    s_red[ty][tx][3] = fy[2];
+    s_red[ty][tx][1] = fy0;
+    s_red[ty][tx][2] = fy1;
+    s_red[ty][tx][3] = fy2;
     s_red[ty][tx][4] = cy;
     __syncthreads();
     if (tid < kTile * NA) {
