@@ -107,17 +107,21 @@ int d4_forward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
                const d4_params* params, double* energy_out, double* q_out,
                void* workspace, size_t workspace_bytes);
 
-/* Gradient of sum_i grad_energy[i] * E_i with respect to the positions,
+/* Gradient of L = sum_i grad_energy[i] * E_i with respect to the positions,
  * written to grad_positions (nbatch, natoms, 3). Must follow d4_forward with
  * the same inputs and workspace. grad_energy NULL means all ones.
- * Replaces the autograd backward of dftd4() (reference
- * test/test_grad/test_pos.py:185-187). */
+ * grad_params: optional device array of 7 doubles that receives dL/ds6,
+ * dL/ds8, dL/ds10, dL/ds9, dL/da1, dL/da2, dL/dalp (summed over the batch;
+ * the s8 and s10 entries are meaningful when those terms are on).
+ * Replaces the autograd backward of dftd4() with respect to the positions
+ * (reference test/test_grad/test_pos.py:185-187) and the damping parameters
+ * (test/test_grad/test_param.py:40-86). */
 int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
                 const int64_t* numbers, const double* positions,
                 const double* charge, const double* q_in,
                 const double* r4r2_in, const d4_params* params,
                 const double* grad_energy, double* grad_positions,
-                void* workspace, size_t workspace_bytes);
+                double* grad_params, void* workspace, size_t workspace_bytes);
 
 /* Fused atom energies and the gradient of the total energy in one pass.
  * Replaces dftd4() followed by torch.autograd.grad(energy.sum(), positions)

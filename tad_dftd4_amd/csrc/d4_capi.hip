@@ -313,8 +313,8 @@ int d4_forward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
 int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
                 const int64_t* numbers, const double* positions, const double* charge,
                 const double* q_in, const double* r4r2_in, const d4_params* params,
-                const double* grad_energy, double* grad_positions, void* workspace,
-                size_t workspace_bytes) {
+                const double* grad_energy, double* grad_positions, double* grad_params,
+                void* workspace, size_t workspace_bytes) {
   int rc = check_common(ctx, nbatch, natoms, numbers, positions, charge, params,
                         workspace, workspace_bytes);
   if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
@@ -323,6 +323,7 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
                      params, workspace);
   c.in.gE = grad_energy;
+  c.p.pgrad = grad_params != nullptr;
   D4_RUN(D4_SLOT_PREP, enter_sorted(c, true), "spatial order");  // the forward's order
   double* g_out = out_atoms(c, grad_positions, c.w.sG);
   D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, g_out),
@@ -333,6 +334,8 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
   D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, g_out), "CN/EEQ chain");
   D4_TRY(scatter_atoms(c, c.w.sG, grad_positions, 3), "scatter gradient");
+  if (grad_params)
+    D4_TRY(d4::launch_pgrad_reduce(c.s, c.p, c.in, c.w, grad_params), "parameter gradients");
   return D4_OK;
 }
 

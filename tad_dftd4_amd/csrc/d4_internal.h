@@ -49,6 +49,7 @@ struct Params {
   int force_tiled;
   int force_sort;
   int force_pairtiles;
+  int pgrad;  // accumulate parameter gradients (backward only)
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).
@@ -91,6 +92,9 @@ struct Work {
   double* sG;       // [B*N*3] gradient in sorted order
   double* gsph;     // [B*NG*4] bounding sphere (centre, radius) of each 16-atom group
   double* apart;    // [B*S*5*N] per-split atom sums of the one-pass three-body kernel
+  double* rt4;      // [B*N] sqrt(r4r2) per atom (parameter gradients)
+  double* pg2;      // [B*N*5] two-body parameter-gradient partials per atom row
+  double* pg3;      // [B*P*4] three-body parameter-gradient partials per work-group
   int64_t B, N;
 };
 
@@ -102,6 +106,15 @@ __host__ __device__ inline int64_t n_groups(int64_t N) { return (N + kGroup - 1)
 // depends on n alone, so a molecule's result does not depend on its batch.
 constexpr int kAtm1MaxN = 256;
 constexpr int kAtm1Acc = 5;  // energy share, gradient (3), dL/dCN
+__host__ __device__ inline int atm1_splits(int n);
+// Three-body parameter-gradient partials per molecule: one per split of the
+// one-pass kernel or one per pair tile of the pair-tile kernel (the larger).
+__host__ __device__ inline int64_t pgrad_partials(int64_t N) {
+  const int64_t nb = n_groups(N);
+  const int64_t tiles = nb * (nb + 1) / 2;
+  const int64_t s = N <= kAtm1MaxN ? atm1_splits((int)N) : 0;
+  return tiles > s ? tiles : s;
+}
 __host__ __device__ inline int atm1_splits(int n) {
   const int nb = (n + kGroup - 1) / kGroup;
   const int tiles = nb * (nb + 1) / 2;
@@ -166,6 +179,9 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
   v.apart = (double*)take(N <= kAtm1MaxN
                               ? 8 * size_t(B) * size_t(atm1_splits((int)N)) * kAtm1Acc * size_t(N)
                               : 0);
+  v.rt4 = (double*)take(8 * bn);
+  v.pg2 = (double*)take(8 * bn * 5);
+  v.pg3 = (double*)take(32 * size_t(B) * size_t(pgrad_partials(N)));
   if (w) *w = v;
   return off;
 }
@@ -204,6 +220,10 @@ hipError_t launch_properties(hipStream_t s, const DevTables& t,
                              const Inputs& in, const Work& w, double* c6_out,
                              double* alpha_out);
 hipError_t launch_copy(hipStream_t s, const double* src, double* dst, int64_t count);
+// Sum of the parameter-gradient partials into out[7] = dL/d(s6, s8, s10, s9,
+// a1, a2, alp) (device pointer).
+hipError_t launch_pgrad_reduce(hipStream_t s, const Params& p, const Inputs& in,
+                               const Work& w, double* out);
 
 // Spatial order (d4_sort.hip). launch_sort computes w.perm (only when
 // `sort`); launch_gather fills the sorted copies of the inputs that are

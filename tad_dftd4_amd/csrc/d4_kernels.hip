@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
 // two-body gradient pieces (one wave per atom row)
 // --------------------------------------------------------------------------
 
-template <bool GRAD>
+template <bool GRAD, bool PG>
 __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
                                                Work w, double* grad_out) {
   const int lane = threadIdx.x & 63;
@@ -201,6 +201,10 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
         grad_out[3 * row + 1] = 0.0;
         grad_out[3 * row + 2] = 0.0;
       }
+      if (PG) {
+        w.rt4[row] = 0.0;
+        for (int k = 0; k < 5; ++k) w.pg2[row * 5 + k] = 0.0;
+      }
     }
     return;
   }
@@ -221,6 +225,8 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
   const double gi = GRAD ? (in.gE ? in.gE[row] : 1.0) : 0.0;
   const double alp3 = p.alp / 3.0;
   double acc_e = 0.0, acc_cn = 0.0, acc_q = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+  // parameter gradients of sum_i gE_i E2_i: d/ds6, d/ds8, d/ds10, d/da1, d/da2
+  double g_s6 = 0.0, g_s8 = 0.0, g_s10 = 0.0, g_a1 = 0.0, g_a2 = 0.0;
   for (int j = lane; j < n; j += 64) {
     const int zj = (int)Z[j];
     if (j == i || zj == 0) {
@@ -275,6 +281,19 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
     de_dc6 *= -0.5;
     const bool in2 = r <= p.cut2;
     if (in2) acc_e += de_dc6 * c6;
+    if (PG && in2) {
+      const double ce = -0.5 * c6 * gi;
+      g_s6 = fma(ce, t6, g_s6);
+      g_s8 = fma(ce * qq, t8, g_s8);
+      // d/dr0 of s6 t6 + s8 qq t8 (+ s10 ...): d t_n / d r0 = -n r0^(n-1) t_n^2
+      double dr0 = p.s6 * (-6.0 * r06 / r0) * t6 * t6 + p.s8 * qq * (-8.0 * r08 / r0) * t8 * t8;
+      if (p.s10on) {
+        g_s10 = fma(ce * qq * qq * (49.0 / 40.0), t10, g_s10);
+        dr0 += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r08 * r0) * t10 * t10;
+      }
+      g_a1 = fma(ce * dr0, sqrt(qq), g_a1);
+      g_a2 = fma(ce, dr0, g_a2);
+    }
     // ATM pair factors (threebody.py:106-145, 241-256, 311-321)
     const double tf = pow(r0 / r, alp3);
     const double pf = sqrt(fabs(c60)) / (r2 * r);
@@ -307,6 +326,13 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
     fy = wave_sum(fy);
     fz = wave_sum(fz);
   }
+  if (PG) {
+    g_s6 = wave_sum(g_s6);
+    g_s8 = wave_sum(g_s8);
+    g_s10 = wave_sum(g_s10);
+    g_a1 = wave_sum(g_a1);
+    g_a2 = wave_sum(g_a2);
+  }
   if (lane == 0) {
     w.e2[row] = acc_e;
     if (GRAD) {
@@ -315,6 +341,14 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
       grad_out[3 * row] = fx;
       grad_out[3 * row + 1] = fy;
       grad_out[3 * row + 2] = fz;
+    }
+    if (PG) {
+      w.rt4[row] = sqrt(r4i);
+      w.pg2[row * 5 + 0] = g_s6;
+      w.pg2[row * 5 + 1] = g_s8;
+      w.pg2[row * 5 + 2] = g_s10;
+      w.pg2[row * 5 + 3] = g_a1;
+      w.pg2[row * 5 + 4] = g_a2;
     }
   }
 }
@@ -348,7 +382,7 @@ __device__ __forceinline__ double sphere_gap(const double* a, const double* b) {
   return sqrt(dx * dx + dy * dy + dz * dz) - a[3] - b[3];
 }
 
-template <bool ENERGY, bool GRAD>
+template <bool ENERGY, bool GRAD, bool PG>
 __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
                                              int tiles_per_mol, int64_t ntiles) {
   __shared__ double s_t[kZB][2 * kTile];
@@ -358,6 +392,8 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
   __shared__ int s_z[kZB];
   __shared__ int s_glist[kMaxGroups];
   __shared__ int s_cnt[kWaves];
+  __shared__ double s_rt[kZB];  // sqrt(r4r2) of the third atoms (PG)
+  __shared__ double s_pg[4][kWaves];
 
   // spread consecutive logical tiles over one XCD: the hardware deals
   // blocks round-robin to the 8 XCDs, so block b runs on XCD b % 8
@@ -418,7 +454,20 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
       if (ENERGY) w.pe[ixy] = w.pe[iyx] = 0.0;
       if (GRAD) w.pg[ixy] = w.pg[iyx] = w.ph[ixy] = w.ph[iyx] = 0.0;
     }
+    if (PG && threadIdx.x < 4) w.pg3[logical * 4 + threadIdx.x] = 0.0;
     return;
+  }
+  // parameter gradients: s9, alp, a1, a2 (each triple is seen from its three
+  // pairs, so the sums are divided by 3 at the end)
+  double g_s9 = 0.0, g_alp = 0.0, g_a1 = 0.0, g_a2 = 0.0;
+  const double a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
+  double rx4 = 0.0, ry4 = 0.0, Axy = 0.0, Bxy = 0.0;
+  if (PG && valid) {
+    rx4 = w.rt4[m * N + x];
+    ry4 = w.rt4[m * N + y];
+    const double sig = sq3 * rx4 * ry4;
+    Bxy = 1.0 / fma(a1p, sig, a2p);
+    Axy = sig * Bxy;
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int total = 0;
@@ -466,6 +515,7 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
       s_r[zl][1] = ok ? R[3 * z + 1] : 0.0;
       s_r[zl][2] = ok ? R[3 * z + 2] : 0.0;
       s_r[zl][3] = (GRAD && ok) ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0;
+      if (PG) s_rt[zl] = ok ? w.rt4[m * N + z] : 0.0;
     }
     __syncthreads();
     if (!valid) continue;
@@ -509,7 +559,33 @@ __global__ __launch_bounds__(256) void k_atm(Params p, Inputs in, Work w,
             s9 * P * f * (g * ia * (T * f * alp - 1.5) + 0.375 * (sa - s * ia) * iabc);
         accg += wgt * dea;
         acch += wgt * en;
+        if (PG) {
+          const double e1 = P * f * g;  // energy without s9
+          const double dT = -6.0 * f * en * wgt;
+          const double TdT = T * dT;
+          g_s9 = fma(wgt, e1, g_s9);
+          g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
+          const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
+          const double bxz = 1.0 / fma(a1p, sxz, a2p), byz = 1.0 / fma(a1p, syz, a2p);
+          g_a1 = fma(TdT * alp / 3.0, Axy + sxz * bxz + syz * byz, g_a1);
+          g_a2 = fma(TdT * alp / 3.0, Bxy + bxz + byz, g_a2);
+        }
       }
+    }
+  }
+  if (PG) {
+    __syncthreads();
+    const double third = 1.0 / 3.0;
+    const double v4[4] = {g_s9 * third, g_alp * third, g_a1 * third, g_a2 * third};
+    for (int k = 0; k < 4; ++k) {
+      const double v = wave_sum(v4[k]);
+      if ((threadIdx.x & 63) == 0) s_pg[k][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      double v = 0.0;
+      for (int q = 0; q < kWaves; ++q) v += s_pg[threadIdx.x][q];
+      w.pg3[logical * 4 + threadIdx.x] = v;
     }
   }
   if (!valid) return;
@@ -625,13 +701,14 @@ struct alignas(32) ZPair {
   double kc;   // d log sqrt|C6(0)| / d CN_c
 };
 
-template <bool ENERGY, bool GRAD>
+template <bool ENERGY, bool GRAD, bool PG>
 __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S,
                                               int64_t nblocks) {
   constexpr int NA = kAtm1Acc;
   constexpr double kFar = 1.0e6;  // position of padding third atoms (bohr)
   __shared__ ZPair s_zc[kTile][2 * kTile + 1];  // [z][x block | y block]
   __shared__ double4 s_r[kTile];                // z position, dL/dE_z
+  __shared__ double s_rt[kTile];                // sqrt(r4r2_z) (parameter gradients)
   __shared__ double s_red[kTile][kTile][NA];
   __shared__ double s_acc[kAtm1MaxN][NA];
 
@@ -654,6 +731,9 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
   const double* pk = w.pk + m * N * N;
   const double c3sq = p.cut3sq, s9 = p.s9, alp = p.alp;
   for (int e = tid; e < n * NA; e += 256) (&s_acc[0][0])[e] = 0.0;
+  // parameter gradients (times 6): s9, alp, a1, a2
+  double g_s9 = 0.0, g_alp = 0.0, g_a1 = 0.0, g_a2 = 0.0;
+  const double a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
 
   for (int t = s; t < tiles; t += Sm) {
     int by = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -686,6 +766,14 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
       }
     }
     const double dxy0 = rx0 - ry0, dxy1 = rx1 - ry1, dxy2 = rx2 - ry2;
+    double rx4 = 0.0, ry4 = 0.0, Axy = 0.0, Bxy = 0.0;
+    if (PG && valid) {
+      rx4 = w.rt4[m * N + x];
+      ry4 = w.rt4[m * N + y];
+      const double sig = sq3 * rx4 * ry4;
+      Bxy = 1.0 / fma(a1p, sig, a2p);
+      Axy = sig * Bxy;
+    }
     double ex = 0.0, ey = 0.0, cx = 0.0, cy = 0.0;
     double fx0 = 0.0, fx1 = 0.0, fx2 = 0.0, fy0 = 0.0, fy1 = 0.0, fy2 = 0.0;
 
@@ -713,6 +801,7 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
         s_r[tid] = ok ? make_double4(R[3 * z], R[3 * z + 1], R[3 * z + 2],
                                      GRAD ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0)
                       : make_double4(kFar, kFar, kFar, 0.0);
+        if (PG) s_rt[tid] = ok ? w.rt4[m * N + z] : 0.0;
       }
       __syncthreads();
 
@@ -773,6 +862,15 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
           cx = fma(We, kxy + zx.kc, cx);
           cy = fma(We, kyx + zy.kc, cy);
           rD = fma(We, zx.kz + zy.kz, rD);
+          if (PG) {
+            const double TdT = -6.0 * f * We * T;  // dL6/dT times T
+            g_s9 = fma(W6, P * f * g, g_s9);
+            g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
+            const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
+            const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
+            g_a1 = fma(TdT * alp / 3.0, Axy + sxz * bxz + syz * byz, g_a1);
+            g_a2 = fma(TdT * alp / 3.0, Bxy + bxz + byz, g_a2);
+          }
         }
         if (ENERGY) rE = row_rotate_down(rE);
         if (GRAD) {
@@ -835,6 +933,22 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
   for (int e = tid; e < NA * n; e += 256) {
     const int c = e / n, i = e - c * n;
     part[c * N + i] = s_acc[i][c];
+  }
+  if (PG) {
+    const double sixth = 1.0 / 6.0;
+    const double v4[4] = {g_s9 * sixth, g_alp * sixth, g_a1 * sixth, g_a2 * sixth};
+    double* red = &s_red[0][0][0];
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+      const double v = wave_sum(v4[k]);
+      if ((tid & 63) == 0) red[k * kWaves + (tid >> 6)] = v;
+    }
+    __syncthreads();
+    if (tid < 4) {
+      double v = 0.0;
+      for (int q = 0; q < kWaves; ++q) v += red[tid * kWaves + q];
+      w.pg3[(m * (int64_t)S + s) * 4 + tid] = v;
+    }
   }
 }
 
@@ -980,6 +1094,41 @@ __global__ __launch_bounds__(256) void k_copy(const double* src, double* dst,
   if (k < count) dst[k] = src[k];
 }
 
+// Parameter gradients: fixed-order sums of the two-body row partials and the
+// three-body work-group partials into dL/d(s6, s8, s10, s9, a1, a2, alp).
+__global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int one_pass,
+                                                      int64_t per_mol, double* out) {
+  __shared__ double red[2 * kWaves];
+  const int tid = threadIdx.x;
+  const int64_t N = w.N;
+  double a[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t row = tid; row < w.B * N; row += 256) {
+    const double* g = w.pg2 + row * 5;
+    a[0] += g[0];
+    a[1] += g[1];
+    a[2] += g[2];
+    a[4] += g[3];
+    a[5] += g[4];
+  }
+  for (int64_t q = tid; q < w.B * per_mol; q += 256) {
+    const int64_t m = q / per_mol;
+    const int k = (int)(q - m * per_mol);
+    const int n = w.natom[m];
+    const int nb = (n + kTile - 1) / kTile;
+    const int valid = one_pass ? atm1_splits(n) : nb * (nb + 1) / 2;
+    if (n == 0 || k >= valid) continue;
+    const double* g = w.pg3 + q * 4;
+    a[3] += g[0];
+    a[6] += g[1];
+    a[4] += g[2];
+    a[5] += g[3];
+  }
+  for (int k = 0; k < 7; ++k) {
+    const double v = block_sum(a[k], red);
+    if (tid == 0) out[k] = v;
+  }
+}
+
 inline unsigned rows_grid(const Work& w) {
   return (unsigned)((w.B * w.N + kWaves - 1) / kWaves);
 }
@@ -1010,12 +1159,15 @@ hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
 hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, bool grad,
                         double* grad_out) {
-  if (grad)
-    hipLaunchKernelGGL(k_pairs<true>, dim3(rows_grid(w)), dim3(256), 0, s, t, p, in, w,
+  if (grad && p.pgrad)
+    hipLaunchKernelGGL((k_pairs<true, true>), dim3(rows_grid(w)), dim3(256), 0, s, t, p, in, w,
                        grad_out);
+  else if (grad)
+    hipLaunchKernelGGL((k_pairs<true, false>), dim3(rows_grid(w)), dim3(256), 0, s, t, p, in,
+                       w, grad_out);
   else
-    hipLaunchKernelGGL(k_pairs<false>, dim3(rows_grid(w)), dim3(256), 0, s, t, p, in, w,
-                       grad_out);
+    hipLaunchKernelGGL((k_pairs<false, false>), dim3(rows_grid(w)), dim3(256), 0, s, t, p, in,
+                       w, grad_out);
   return hipGetLastError();
 }
 
@@ -1025,14 +1177,17 @@ hipError_t launch_atm1(hipStream_t s, const Params& p, const Inputs& in, const W
   const int64_t nblocks = w.B * S;
   const int64_t grid = (nblocks + 7) / 8 * 8;  // multiple of 8 for the XCD remap
   if (energy && grad)
-    hipLaunchKernelGGL((k_atm1<true, true>), dim3((unsigned)grid), dim3(256), 0, s, p, in, w,
-                       S, nblocks);
+    hipLaunchKernelGGL((k_atm1<true, true, false>), dim3((unsigned)grid), dim3(256), 0, s, p,
+                       in, w, S, nblocks);
   else if (energy)
-    hipLaunchKernelGGL((k_atm1<true, false>), dim3((unsigned)grid), dim3(256), 0, s, p, in, w,
-                       S, nblocks);
+    hipLaunchKernelGGL((k_atm1<true, false, false>), dim3((unsigned)grid), dim3(256), 0, s, p,
+                       in, w, S, nblocks);
+  else if (grad && p.pgrad)
+    hipLaunchKernelGGL((k_atm1<false, true, true>), dim3((unsigned)grid), dim3(256), 0, s, p,
+                       in, w, S, nblocks);
   else if (grad)
-    hipLaunchKernelGGL((k_atm1<false, true>), dim3((unsigned)grid), dim3(256), 0, s, p, in, w,
-                       S, nblocks);
+    hipLaunchKernelGGL((k_atm1<false, true, false>), dim3((unsigned)grid), dim3(256), 0, s, p,
+                       in, w, S, nblocks);
   return hipGetLastError();
 }
 
@@ -1047,13 +1202,16 @@ hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
   hipError_t e = launch_groups(s, in, w);
   if (e != hipSuccess) return e;
   if (energy && grad)
-    hipLaunchKernelGGL((k_atm<true, true>), dim3((unsigned)grid), dim3(256), 0, s, p,
+    hipLaunchKernelGGL((k_atm<true, true, false>), dim3((unsigned)grid), dim3(256), 0, s, p,
                        in, w, tiles, ntiles);
   else if (energy)
-    hipLaunchKernelGGL((k_atm<true, false>), dim3((unsigned)grid), dim3(256), 0, s, p,
+    hipLaunchKernelGGL((k_atm<true, false, false>), dim3((unsigned)grid), dim3(256), 0, s, p,
+                       in, w, tiles, ntiles);
+  else if (grad && p.pgrad)
+    hipLaunchKernelGGL((k_atm<false, true, true>), dim3((unsigned)grid), dim3(256), 0, s, p,
                        in, w, tiles, ntiles);
   else if (grad)
-    hipLaunchKernelGGL((k_atm<false, true>), dim3((unsigned)grid), dim3(256), 0, s, p,
+    hipLaunchKernelGGL((k_atm<false, true, false>), dim3((unsigned)grid), dim3(256), 0, s, p,
                        in, w, tiles, ntiles);
   return hipGetLastError();
 }
@@ -1097,6 +1255,16 @@ hipError_t launch_properties(hipStream_t s, const DevTables& t, const Inputs& in
                              const Work& w, double* c6_out, double* alpha_out) {
   hipLaunchKernelGGL(k_properties, dim3(rows_grid(w)), dim3(256), 0, s, t, in, w,
                      c6_out, alpha_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pgrad_reduce(hipStream_t s, const Params& p, const Inputs& in,
+                               const Work& w, double* out) {
+  const bool one_pass = w.N <= kAtm1MaxN && !p.force_pairtiles;
+  const int64_t nb = n_groups(w.N);
+  const int64_t per_mol = one_pass ? atm1_splits((int)w.N) : nb * (nb + 1) / 2;
+  hipLaunchKernelGGL(k_pgrad_reduce, dim3(1), dim3(256), 0, s, in, w, one_pass ? 1 : 0,
+                     per_mol, out);
   return hipGetLastError();
 }
 

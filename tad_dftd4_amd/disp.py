@@ -13,7 +13,10 @@ from hand-written backward kernels through ``torch.autograd``.
 Not supported (raise ``NotImplementedError``; there is no fallback):
 non-default ``cn_function``/``counting_function``/``damping_function``,
 models other than D4 with EEQ reference charges, gradients with respect to
-damping parameters or user-supplied charges, double backward.
+user-supplied charges, double backward. Gradients with respect to the damping
+parameters s6, s8, s10, s9, a1, a2 and alp (0-d tensors with
+``requires_grad``) come from the same backward kernels
+(``test/test_grad/test_param.py:40-86``).
 """
 
 from __future__ import annotations
@@ -81,22 +84,35 @@ def kernel_params(param: Param, cutoff: Cutoff | None, model: D4Model | None):
 
 
 class _D4Energy(torch.autograd.Function):
-    """Atom energies with a hand-written backward (HIP kernels)."""
+    """Atom energies with a hand-written backward (HIP kernels). The trailing
+    inputs are the damping-parameter tensors named by ``pkeys`` (a subset of
+    ``runtime.PARAM_ORDER``) whose gradients are requested."""
 
     @staticmethod
-    def forward(ctx, positions, numbers, charge, q, r4r2, kp):
+    def forward(ctx, positions, numbers, charge, q, r4r2, kp, pkeys, *ptensors):
         energy, ws = runtime.forward(numbers, positions, charge, kp, q=q, r4r2=r4r2)
         ctx.save_for_backward(positions, numbers, charge, q, r4r2, ws)
         ctx.kp = kp
+        ctx.pkeys = pkeys
+        ctx.pmeta = [(t.shape, t.dtype) for t in ptensors]
         return energy
 
     @staticmethod
     @torch.autograd.function.once_differentiable
     def backward(ctx, grad_energy):
         positions, numbers, charge, q, r4r2, ws = ctx.saved_tensors
-        grad = runtime.backward(numbers, positions, charge, ctx.kp, ws,
-                                grad_energy.contiguous(), q=q, r4r2=r4r2)
-        return grad, None, None, None, None, None
+        want = any(ctx.needs_input_grad[7:])
+        out = runtime.backward(numbers, positions, charge, ctx.kp, ws,
+                               grad_energy.contiguous(), q=q, r4r2=r4r2, params=want)
+        grad, gpar = out if want else (out, None)
+        pgrads = []
+        for key, (shape, dtype) in zip(ctx.pkeys, ctx.pmeta):
+            if gpar is None:
+                pgrads.append(None)
+            else:
+                pgrads.append(gpar[runtime.PARAM_ORDER.index(key)].reshape(shape).to(dtype))
+        grad_pos = grad if ctx.needs_input_grad[0] else None
+        return (grad_pos, None, None, None, None, None, None, *pgrads)
 
 
 def _prepare(numbers, positions, charge, model, rcov, r4r2, rvdw, q, cn_function,
@@ -188,7 +204,14 @@ def dftd4(
         counting_function, damping_function,
     )
     kp = kernel_params(param, cutoff, model_inst)
-    energy = _D4Energy.apply(pos, nums, chg, qq, rr, kp)
+    pkeys = tuple(
+        k for k in runtime.PARAM_ORDER
+        if isinstance(param.get(k), torch.Tensor) and param[k].requires_grad
+    )
+    if pkeys and (not torch.is_grad_enabled()):
+        pkeys = ()
+    ptensors = tuple(param[k] for k in pkeys)
+    energy = _D4Energy.apply(pos, nums, chg, qq, rr, kp, pkeys, *ptensors)
     return energy.reshape(*lead, n).to(positions.dtype)
 
 
