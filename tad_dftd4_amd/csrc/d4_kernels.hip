@@ -743,8 +743,8 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
     const int x = bx * kTile + tx, y = by * kTile + ty;
     const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
     double rx0 = 0.0, rx1 = 0.0, rx2 = 0.0, ry0 = 0.0, ry1 = 0.0, ry2 = 0.0;
-    double a = 1.0, ia = 1.0, txy = 0.0, pxy = 0.0, kxy = 0.0, kyx = 0.0, gx = 0.0, gy = 0.0;
-    double mxy = 0.0;
+    double a = 1.0, ia = 1.0, txy = 0.0, pxy = 0.0, pxy0 = 0.0, kxy = 0.0, kyx = 0.0;
+    double gx = 0.0, gy = 0.0, mxy = 0.0;
     if (valid) {
       rx0 = R[3 * x];
       rx1 = R[3 * x + 1];
@@ -756,7 +756,8 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
       a = d0 * d0 + d1 * d1 + d2 * d2;
       ia = 1.0 / a;
       txy = pt[(int64_t)x * N + y];
-      pxy = pp[(int64_t)x * N + y];
+      pxy0 = pp[(int64_t)x * N + y];
+      pxy = s9 * pxy0;  // s9 folded into the pair factor
       mxy = a <= c3sq ? 1.0 : 0.0;
       if (GRAD) {
         kxy = pk[(int64_t)x * N + y];
@@ -821,6 +822,7 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
         const double cc = on != 0.0 ? c2 : 1.0;  // r_yz^2
         const double T = txy * zx.t * zy.t;
         const double P = on * pxy * zx.p * zy.p;
+        const double P0 = PG ? on * pxy0 * zx.p * zy.p : 0.0;  // without s9
         const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
         const double sabc = X * Y * W3;
         const double bc = bb * cc;
@@ -829,11 +831,13 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
         const double q0 = fast_rcp(abc * opT);
         const double f = abc * q0;     // 1 / (1 + 6 T)
         const double iabc = opT * q0;  // 1 / (a b c)
-        const double g = fma(0.375 * sabc, iabc, 1.0);
-        const double K = s9 * P * f;
+        const double G2 = 0.375 * iabc;
+        const double g = fma(G2, sabc, 1.0);
+        const double K = P * f;  // s9 is in P
         const double e = K * g;
         const double mxz = bb <= c3sq ? 1.0 : 0.0, myz = cc <= c3sq ? 1.0 : 0.0;
-        const double sx = myz * (mxy + mxz), sy = mxz * (mxy + myz), sz = mxy * (mxz + myz);
+        const double m2 = mxz * myz;
+        const double sx = fma(myz, mxy, m2), sy = fma(mxz, mxy, m2), sz = mxy * (mxz + myz);
         if (ENERGY) {
           ex = fma(sx, e, ex);
           ey = fma(sy, e, ey);
@@ -842,14 +846,16 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
         if (GRAD) {
           const double W6 = fma(gx, sx, fma(gy, sy, rz.w * sz));
           const double We = W6 * e;
-          const double G2 = 0.375 * iabc;
           const double H = fma(g, fma(alp * T, f, -1.5), -G2 * sabc);
-          const double ibc = a * iabc;  // 1 / (b c)
-          const double XY = X * Y, XW = X * W3, YW = Y * W3;
-          const double KW = 2.0 * K * W6;
-          const double ca = KW * fma(H, ia, G2 * (YW + XW - XY));
-          const double cb = KW * fma(H, cc * ibc, G2 * (YW - XW + XY));
-          const double cg = KW * fma(H, bb * ibc, G2 * (XW + XY - YW));
+          // ds/da, ds/db, ds/dc from A = Y W3, B = X W3, C = X Y
+          const double A = Y * W3, Bq = X * W3, C = X * Y;
+          const double Ssum = A + Bq + C;
+          const double KW = K * W6;  // half of dL6/de-weighted factor; x2 at the end
+          const double al = KW * H, be = KW * G2;
+          const double ali = al * a * iabc;  // al / (b c)
+          const double ca = fma(al, ia, be * fma(-2.0, C, Ssum));
+          const double cb = fma(ali, cc, be * fma(-2.0, Bq, Ssum));
+          const double cg = fma(ali, bb, be * fma(-2.0, A, Ssum));
           fx0 = fma(ca, dxy0, fma(cb, u0, fx0));
           fx1 = fma(ca, dxy1, fma(cb, u1, fx1));
           fx2 = fma(ca, dxy2, fma(cb, u2, fx2));
@@ -864,7 +870,7 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
           rD = fma(We, zx.kz + zy.kz, rD);
           if (PG) {
             const double TdT = -6.0 * f * We * T;  // dL6/dT times T
-            g_s9 = fma(W6, P * f * g, g_s9);
+            g_s9 = fma(W6, P0 * f * g, g_s9);
             g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
             const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
             const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
@@ -880,11 +886,12 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
           rD = row_rotate_down(rD);
         }
       }
-      // lane tx of every row now holds that row's sums for z0 + tx
+      // lane tx of every row now holds that row's sums for z0 + tx (the
+      // gradient parts were accumulated at half weight)
       s_red[ty][tx][0] = rE;
-      s_red[ty][tx][1] = rF0;
-      s_red[ty][tx][2] = rF1;
-      s_red[ty][tx][3] = rF2;
+      s_red[ty][tx][1] = 2.0 * rF0;
+      s_red[ty][tx][2] = 2.0 * rF1;
+      s_red[ty][tx][3] = 2.0 * rF2;
       s_red[ty][tx][4] = rD;
       __syncthreads();
       if (tid < kTile * NA) {
@@ -899,9 +906,9 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
     // tile sums of the x atoms (over the rows) and the y atoms (over the columns)
     __syncthreads();
     s_red[ty][tx][0] = ex;
-    s_red[ty][tx][1] = fx0;
-    s_red[ty][tx][2] = fx1;
-    s_red[ty][tx][3] = fx2;
+    s_red[ty][tx][1] = 2.0 * fx0;
+    s_red[ty][tx][2] = 2.0 * fx1;
+    s_red[ty][tx][3] = 2.0 * fx2;
     s_red[ty][tx][4] = cx;
     __syncthreads();
     if (tid < kTile * NA) {
@@ -914,9 +921,9 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
     }
     __syncthreads();
     s_red[ty][tx][0] = ey;
-    s_red[ty][tx][1] = fy0;
-    s_red[ty][tx][2] = fy1;
-    s_red[ty][tx][3] = fy2;
+    s_red[ty][tx][1] = 2.0 * fy0;
+    s_red[ty][tx][2] = 2.0 * fy1;
+    s_red[ty][tx][3] = 2.0 * fy2;
     s_red[ty][tx][4] = cy;
     __syncthreads();
     if (tid < kTile * NA) {
