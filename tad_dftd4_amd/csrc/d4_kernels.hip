@@ -694,7 +694,7 @@ __device__ __forceinline__ double fast_rcp(double d) {
 }
 
 // Pair data of (z, c) staged for the one-pass kernel, c an atom of the tile.
-struct alignas(32) ZPair {
+struct ZPair {
   double t;    // (R0 / r)^(alp / 3)
   double p;    // sqrt|C6(q = 0)| / r^3
   double kz;   // d log sqrt|C6(0)| / d CN_z
@@ -706,8 +706,13 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
                                               int64_t nblocks) {
   constexpr int NA = kAtm1Acc;
   constexpr double kFar = 1.0e6;  // position of padding third atoms (bohr)
-  __shared__ ZPair s_zc[kTile][2 * kTile + 1];  // [z][x block | y block]
-  __shared__ double4 s_r[kTile];                // z position, dL/dE_z
+  // [z][x block | y block] pair records. The row stride is 67 16-byte units
+  // (odd): the 16 lanes of a row read 16 different rows z on the skewed
+  // schedule, and an odd stride keeps their x- and y-column reads on
+  // distinct LDS banks.
+  constexpr int kZcRow = 134;                   // doubles per z row (33 records + 16 B)
+  __shared__ __align__(16) double s_zc[kTile * kZcRow];
+  __shared__ __align__(16) double s_r[kTile * 6];  // z position, dL/dE_z (48-byte stride)
   __shared__ double s_rt[kTile];                // sqrt(r4r2_z) (parameter gradients)
   __shared__ double s_red[kTile][kTile][NA];
   __shared__ double s_acc[kAtm1MaxN][NA];
@@ -794,14 +799,18 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
             v.kc = pk[(int64_t)col * N + z];
           }
         }
-        s_zc[zl][c] = v;
+        double2* dst = reinterpret_cast<double2*>(s_zc + zl * kZcRow + 4 * c);
+        dst[0] = make_double2(v.t, v.p);
+        dst[1] = make_double2(v.kz, v.kc);
       }
       if (tid < kTile) {
         const int z = z0 + tid;
         const bool ok = z < n && Z[z] != 0;
-        s_r[tid] = ok ? make_double4(R[3 * z], R[3 * z + 1], R[3 * z + 2],
-                                     GRAD ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0)
-                      : make_double4(kFar, kFar, kFar, 0.0);
+        double* r = s_r + 6 * tid;
+        r[0] = ok ? R[3 * z] : kFar;
+        r[1] = ok ? R[3 * z + 1] : kFar;
+        r[2] = ok ? R[3 * z + 2] : kFar;
+        r[3] = (GRAD && ok) ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0;
         if (PG) s_rt[tid] = ok ? w.rt4[m * N + z] : 0.0;
       }
       __syncthreads();
@@ -810,8 +819,16 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
 #pragma unroll 2
       for (int k = 0; k < kTile; ++k) {
         const int zl = (k + tx) & (kTile - 1);
-        const double4 rz = s_r[zl];
-        const ZPair zx = s_zc[zl][tx], zy = s_zc[zl][kTile + ty];
+        const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
+        const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
+        const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
+        const double2* px = reinterpret_cast<const double2*>(s_zc + zl * kZcRow + 4 * tx);
+        const double2* py = reinterpret_cast<const double2*>(s_zc + zl * kZcRow + 4 * (kTile + ty));
+        const double2 x01 = px[0], y01 = py[0];
+        const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
+        const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
+        const ZPair zx = {x01.x, x01.y, x23.x, x23.y};
+        const ZPair zy = {y01.x, y01.y, y23.x, y23.y};
         // third atoms up to y, outside the molecule or padding carry no weight
         const double on = (valid && z0 + zl > y) ? 1.0 : 0.0;
         const double u0 = rx0 - rz.x, u1 = rx1 - rz.y, u2 = rx2 - rz.z;
