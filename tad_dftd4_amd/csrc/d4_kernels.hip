@@ -15,6 +15,18 @@
 namespace d4 {
 namespace {
 
+// Row kernels give a group of RL lanes to one atom row (RL = 16, 32 or 64,
+// chosen from the padded length): short rows keep the lanes busy with a
+// quarter wave per row, long rows take a full wave.
+template <int RL>
+__device__ __forceinline__ double row_sum(double v) {
+#pragma unroll
+  for (int off = RL / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, RL);
+  return v;
+}
+
+inline int row_lanes(int64_t N) { return N <= 160 ? 16 : (N <= 384 ? 32 : 64); }
+
 // --------------------------------------------------------------------------
 // prep: molecule extents and atomic-number validation (one wave / molecule)
 // --------------------------------------------------------------------------
@@ -43,10 +55,11 @@ __global__ __launch_bounds__(256) void k_prep(const int64_t* __restrict__ number
 // coordination numbers: D4 (EN-weighted) and EEQ (capped), one wave / atom
 // --------------------------------------------------------------------------
 
+template <int RL>
 __global__ __launch_bounds__(256) void k_cn(DevTables t, Params p, Inputs in,
                                             Work w) {
-  const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & (RL - 1);
+  const int64_t row = (int64_t)blockIdx.x * (256 / RL) + threadIdx.x / RL;
   if (row >= w.B * w.N) return;
   const int64_t m = row / w.N;
   const int i = (int)(row - m * w.N);
@@ -65,7 +78,7 @@ __global__ __launch_bounds__(256) void k_cn(DevTables t, Params p, Inputs in,
   const double xi = R[3 * i], yi = R[3 * i + 1], zzi = R[3 * i + 2];
   const double rci = t.rcov[zi], eni = t.en[zi];
   double s4 = 0.0, se = 0.0;
-  for (int j = lane; j < n; j += 64) {
+  for (int j = lane; j < n; j += RL) {
     const int zj = (int)Z[j];
     if (j == i || zj == 0) continue;
     const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
@@ -74,8 +87,8 @@ __global__ __launch_bounds__(256) void k_cn(DevTables t, Params p, Inputs in,
     if (r <= p.cutcn) s4 += en_factor(eni, t.en[zj]) * c;
     if (r <= p.cuteeq) se += c;
   }
-  s4 = wave_sum(s4);
-  se = wave_sum(se);
+  s4 = row_sum<RL>(s4);
+  se = row_sum<RL>(se);
   if (lane == 0) {
     w.cn[row] = s4;
     // smooth cap: log(1 + e^cmax) - log(1 + e^(cmax - cn))
@@ -170,11 +183,11 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
 // two-body gradient pieces (one wave per atom row)
 // --------------------------------------------------------------------------
 
-template <bool GRAD, bool PG>
+template <bool GRAD, bool PG, int RL>
 __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
                                                Work w, double* grad_out) {
-  const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & (RL - 1);
+  const int64_t row = (int64_t)blockIdx.x * (256 / RL) + threadIdx.x / RL;
   if (row >= w.B * w.N) return;
   const int64_t N = w.N;
   const int64_t m = row / N;
@@ -187,7 +200,7 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
   double* pkr = w.pk + row * N;
   const int zi = i < n ? (int)Z[i] : 0;
   if (zi == 0) {
-    for (int j = lane; j < n; j += 64) {
+    for (int j = lane; j < n; j += RL) {
       ptr[j] = 0.0;
       ppr[j] = 0.0;
       if (GRAD) pkr[j] = 0.0;
@@ -227,7 +240,7 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
   double acc_e = 0.0, acc_cn = 0.0, acc_q = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
   // parameter gradients of sum_i gE_i E2_i: d/ds6, d/ds8, d/ds10, d/da1, d/da2
   double g_s6 = 0.0, g_s8 = 0.0, g_s10 = 0.0, g_a1 = 0.0, g_a2 = 0.0;
-  for (int j = lane; j < n; j += 64) {
+  for (int j = lane; j < n; j += RL) {
     const int zj = (int)Z[j];
     if (j == i || zj == 0) {
       ptr[j] = 0.0;
@@ -318,20 +331,20 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
       }
     }
   }
-  acc_e = wave_sum(acc_e);
+  acc_e = row_sum<RL>(acc_e);
   if (GRAD) {
-    acc_cn = wave_sum(acc_cn);
-    acc_q = wave_sum(acc_q);
-    fx = wave_sum(fx);
-    fy = wave_sum(fy);
-    fz = wave_sum(fz);
+    acc_cn = row_sum<RL>(acc_cn);
+    acc_q = row_sum<RL>(acc_q);
+    fx = row_sum<RL>(fx);
+    fy = row_sum<RL>(fy);
+    fz = row_sum<RL>(fz);
   }
   if (PG) {
-    g_s6 = wave_sum(g_s6);
-    g_s8 = wave_sum(g_s8);
-    g_s10 = wave_sum(g_s10);
-    g_a1 = wave_sum(g_a1);
-    g_a2 = wave_sum(g_a2);
+    g_s6 = row_sum<RL>(g_s6);
+    g_s8 = row_sum<RL>(g_s8);
+    g_s10 = row_sum<RL>(g_s10);
+    g_a1 = row_sum<RL>(g_a1);
+    g_a2 = row_sum<RL>(g_a2);
   }
   if (lane == 0) {
     w.e2[row] = acc_e;
@@ -1014,10 +1027,11 @@ __global__ __launch_bounds__(256) void k_atm1_rows(Inputs in, Work w, double* en
 // matrix (one wave per atom row)
 // --------------------------------------------------------------------------
 
+template <int RL>
 __global__ __launch_bounds__(256) void k_chain(DevTables t, Params p, Inputs in,
                                                Work w, double* grad_out) {
-  const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & (RL - 1);
+  const int64_t row = (int64_t)blockIdx.x * (256 / RL) + threadIdx.x / RL;
   if (row >= w.B * w.N) return;
   const int64_t N = w.N;
   const int64_t m = row / N;
@@ -1032,7 +1046,7 @@ __global__ __launch_bounds__(256) void k_chain(DevTables t, Params p, Inputs in,
   const double rci = t.rcov[zi], eni = t.en[zi], radi = t.rad[zi];
   const double dcni = w.dldcn[row], bi = w.beta[row], li = w.lam[row], qi = w.q[row];
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  for (int j = lane; j < n; j += 64) {
+  for (int j = lane; j < n; j += RL) {
     const int zj = (int)Z[j];
     if (j == i || zj == 0) continue;
     const int64_t rj = m * N + j;
@@ -1054,9 +1068,9 @@ __global__ __launch_bounds__(256) void k_chain(DevTables t, Params p, Inputs in,
     fy += coef * dy;
     fz += coef * dz;
   }
-  fx = wave_sum(fx);
-  fy = wave_sum(fy);
-  fz = wave_sum(fz);
+  fx = row_sum<RL>(fx);
+  fy = row_sum<RL>(fy);
+  fz = row_sum<RL>(fz);
   if (lane == 0) {
     grad_out[3 * row] += fx;
     grad_out[3 * row + 1] += fy;
@@ -1153,6 +1167,25 @@ __global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int one
   }
 }
 
+inline unsigned rows_grid_rl(const Work& w, int rl) {
+  const int per = 256 / rl;
+  return (unsigned)((w.B * w.N + per - 1) / per);
+}
+
+// Launch a row kernel instantiated for the row width of this batch (the
+// three instantiations are passed in parentheses).
+#define D4_LAUNCH_ROWS(K16, K32, K64, s, w, ...)                  \
+  do {                                                           \
+    const int rl_ = row_lanes((w).N);                           \
+    const dim3 g_(rows_grid_rl((w), rl_));                       \
+    if (rl_ == 16)                                               \
+      hipLaunchKernelGGL(K16, g_, dim3(256), 0, s, __VA_ARGS__); \
+    else if (rl_ == 32)                                          \
+      hipLaunchKernelGGL(K32, g_, dim3(256), 0, s, __VA_ARGS__); \
+    else                                                         \
+      hipLaunchKernelGGL(K64, g_, dim3(256), 0, s, __VA_ARGS__); \
+  } while (0)
+
 inline unsigned rows_grid(const Work& w) {
   return (unsigned)((w.B * w.N + kWaves - 1) / kWaves);
 }
@@ -1169,7 +1202,7 @@ hipError_t launch_prep(hipStream_t s, const Inputs& in, const Work& w) {
 
 hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
                      const Inputs& in, const Work& w) {
-  hipLaunchKernelGGL(k_cn, dim3(rows_grid(w)), dim3(256), 0, s, t, p, in, w);
+  D4_LAUNCH_ROWS((k_cn<16>), (k_cn<32>), (k_cn<64>), s, w, t, p, in, w);
   return hipGetLastError();
 }
 
@@ -1184,14 +1217,14 @@ hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, bool grad,
                         double* grad_out) {
   if (grad && p.pgrad)
-    hipLaunchKernelGGL((k_pairs<true, true>), dim3(rows_grid(w)), dim3(256), 0, s, t, p, in, w,
-                       grad_out);
+    D4_LAUNCH_ROWS((k_pairs<true, true, 16>), (k_pairs<true, true, 32>),
+                   (k_pairs<true, true, 64>), s, w, t, p, in, w, grad_out);
   else if (grad)
-    hipLaunchKernelGGL((k_pairs<true, false>), dim3(rows_grid(w)), dim3(256), 0, s, t, p, in,
-                       w, grad_out);
+    D4_LAUNCH_ROWS((k_pairs<true, false, 16>), (k_pairs<true, false, 32>),
+                   (k_pairs<true, false, 64>), s, w, t, p, in, w, grad_out);
   else
-    hipLaunchKernelGGL((k_pairs<false, false>), dim3(rows_grid(w)), dim3(256), 0, s, t, p, in,
-                       w, grad_out);
+    D4_LAUNCH_ROWS((k_pairs<false, false, 16>), (k_pairs<false, false, 32>),
+                   (k_pairs<false, false, 64>), s, w, t, p, in, w, grad_out);
   return hipGetLastError();
 }
 
@@ -1270,8 +1303,7 @@ hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, con
 
 hipError_t launch_chain(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, double* grad_out) {
-  hipLaunchKernelGGL(k_chain, dim3(rows_grid(w)), dim3(256), 0, s, t, p, in, w,
-                     grad_out);
+  D4_LAUNCH_ROWS((k_chain<16>), (k_chain<32>), (k_chain<64>), s, w, t, p, in, w, grad_out);
   return hipGetLastError();
 }
 
