@@ -68,6 +68,7 @@ class Tables:
         self.clsq = _t(r["clsq"], dtype)
         self.clsh = _t(r["clsh"], dtype)
         self.cp_weights = _t(r["cp_weights"], dtype)
+        self.wfpair = _t(r["wfpair"], dtype)
         raw = _t(r["r4_over_r2"], dtype)
         # data/r4r2.py:84-88
         sqrtz = torch.sqrt(torch.arange(len(raw), dtype=dtype))
@@ -403,6 +404,114 @@ def get_atomic_c6(rc6: Tensor, gw: Tensor) -> Tensor:
     return torch.einsum("...ijab,...ia,...jb->...ij", rc6, gw, gw)
 
 
+class _StableRatioPair(torch.autograd.Function):
+    """expw / norm of the D4S weights with the reference's forward values and
+    a backward that stays finite where the norm underflows (as
+    ``_StableRatio``, with the pairwise factor inside the exponential:
+    tmp = exp(-wf dcn^2), expw = sum_k tmp^k)."""
+
+    @staticmethod
+    def forward(ctx, cn, refcn, refc, mask, wf, exceptional):
+        dcn = cn.type(torch.double).unsqueeze(-1).unsqueeze(-3) - refcn
+        tmp = torch.where(mask, torch.exp(-dcn * dcn * wf.unsqueeze(-1)),
+                          torch.tensor(0.0, dtype=torch.double))
+        zero = torch.tensor(0.0, dtype=torch.double)
+
+        def series(n, deriv):
+            out = torch.zeros_like(tmp)
+            for i in range(1, n + 1):
+                term = torch.pow(tmp, i)
+                out = out + (-2.0 * i * wf.unsqueeze(-1) * dcn * term if deriv else term)
+            return out
+
+        expw = torch.where(mask, torch.where(refc == 3, series(3, False),
+                                             torch.where(refc == 1, series(1, False), tmp)),
+                           zero)
+        dexpw = torch.where(mask, torch.where(refc == 3, series(3, True),
+                                              torch.where(refc == 1, series(1, True), zero)),
+                            zero)
+        norm = torch.where(mask, torch.sum(expw, dim=-1, keepdim=True),
+                           torch.tensor(1e-300, dtype=torch.double))
+        w = expw / norm
+        dnorm = torch.sum(dexpw, dim=-1, keepdim=True)
+        dw = torch.where(exceptional | ~mask, zero,
+                         (dexpw - torch.where(exceptional, zero, w) * dnorm) / norm)
+        ctx.save_for_backward(dw)
+        return w
+
+    @staticmethod
+    def backward(ctx, grad):
+        (dw,) = ctx.saved_tensors
+        # d w[m, n, r] / d cn[n]: sum over the partner m and the references
+        return (grad * dw).sum(dim=(-3, -1)).type(grad.dtype), None, None, None, None, None
+
+
+def weight_references_d4s(numbers: Tensor, cn: Tensor | None = None,
+                          q: Tensor | None = None, dtype=torch.float64,
+                          nan_safe: bool = False) -> Tensor:
+    """``D4SModel.weight_references`` (``model/d4s.py:114-263``), eeq reference
+    charges: Gaussian weights of atom n with the pairwise factor
+    wf(Z_n, Z_m) (``data/wfpair.py``) for every partner m; returns
+    (..., m, n, ref)."""
+    t = tables(dtype)
+    if cn is None:
+        cn = torch.zeros(numbers.shape, dtype=dtype)
+    if q is None:
+        q = torch.zeros(numbers.shape, dtype=dtype)
+    refq = t.clsq[numbers]
+    zero = torch.tensor(0.0, dtype=dtype)
+    zero_double = torch.tensor(0.0, dtype=torch.double)
+    _refc = t.refc[numbers]
+    shp = (*_refc.shape[:-1], _refc.shape[-2], _refc.shape[-1])
+    refc = _refc.unsqueeze(-3).expand(*shp)
+    mask = refc > 0
+    refcn = t.refcovcn[numbers].to(torch.double).unsqueeze(-3).expand(*shp)
+    dcn = cn.type(torch.double).unsqueeze(-1).unsqueeze(-3) - refcn
+    wf = t.wfpair.to(torch.double)[numbers.unsqueeze(-1), numbers.unsqueeze(-2)]
+    # (..., m, n, r) * (..., m, n, r) * (..., n, m) -> (..., m, n, r)
+    arg = torch.einsum("...mnr,...mnr,...nm->...mnr", -dcn, dcn, wf)
+    tmp = torch.where(mask, torch.exp(arg), zero_double)
+
+    def refc_pow(n: int) -> Tensor:
+        return sum((torch.pow(tmp, i) for i in range(1, n + 1)), zero_double)
+
+    refc_pow_1 = torch.where(refc == 1, refc_pow(1), tmp)
+    refc_pow_final = torch.where(refc == 3, refc_pow(3), refc_pow_1)
+    expw = torch.where(mask, refc_pow_final, zero_double)
+    norm = torch.where(mask, torch.sum(expw, dim=-1, keepdim=True),
+                       torch.tensor(1e-300, dtype=torch.double))
+    gw_temp = (expw / norm).type(dtype)
+    maxcn = torch.max(refcn, dim=-1, keepdim=True)[0]
+    exceptional = torch.isnan(gw_temp) | (gw_temp > torch.finfo(dtype).max)
+    if nan_safe:
+        gw_temp = _StableRatioPair.apply(cn, refcn, refc, mask, wf.transpose(-1, -2),
+                                         exceptional).type(dtype)
+    gw = torch.where(
+        exceptional,
+        torch.where(refcn == maxcn, torch.tensor(1.0, dtype=dtype), zero),
+        gw_temp,
+    )
+    zeff = t.zeff[numbers].unsqueeze(-1)
+    gam = t.gam[numbers].unsqueeze(-1) * GC
+    _z = zeta(gam, refq + zeff, q.unsqueeze(-1) + zeff)
+    z = torch.where(mask, _z.unsqueeze(-3).expand(*shp), zero)
+    return z * gw
+
+
+def get_atomic_c6_d4s(rc6: Tensor, gw: Tensor) -> Tensor:
+    """``D4SModel.get_atomic_c6`` (``model/d4s.py:265-285``): gw[j, i, a] is
+    atom i's weight in the pair with j."""
+    return torch.einsum("...ijab,...jia,...ijb->...ij", rc6, gw, gw)
+
+
+def _model_fns(model: str):
+    if model == "d4":
+        return weight_references, get_atomic_c6
+    if model == "d4s":
+        return weight_references_d4s, get_atomic_c6_d4s
+    raise ValueError(f"unknown model {model}")
+
+
 def get_polarizabilities(numbers: Tensor, gw: Tensor) -> Tensor:
     """``BaseModel.get_polarizabilities`` (``model/base.py:286-302``)."""
     alpha = get_alpha(numbers, gw.dtype)
@@ -515,13 +624,15 @@ def atm_dispersion(numbers: Tensor, positions: Tensor, c9: Tensor,
 
 
 def atm_term(numbers: Tensor, positions: Tensor, param: dict, cn: Tensor,
-             rc6: Tensor, r4r2: Tensor, cutoff: float, nan_safe: bool = False) -> Tensor:
+             rc6: Tensor, r4r2: Tensor, cutoff: float, nan_safe: bool = False,
+             model: str = "d4") -> Tensor:
     """``D4ATMApprox.calculate``: ``C9ApproxMixin.get_c9`` with q = None
     (``threebody.py:311-321``), ``RadiiBJMixin.get_radii``
     (``threebody.py:244-256``), then ``get_atm_dispersion``."""
     dtype = positions.dtype
-    gw = weight_references(numbers, cn, None, dtype=dtype, nan_safe=nan_safe)
-    c6 = get_atomic_c6(rc6, gw)
+    wref, c6fn = _model_fns(model)
+    gw = wref(numbers, cn, None, dtype=dtype, nan_safe=nan_safe)
+    c6 = c6fn(rc6, gw)
     c9 = safe_sqrt(torch.abs(c6.unsqueeze(-1) * c6.unsqueeze(-2) * c6.unsqueeze(-3)))
     a1 = param.get("a1", torch.tensor(A1, dtype=dtype))
     a2 = param.get("a2", torch.tensor(A2, dtype=dtype))
@@ -533,10 +644,11 @@ def atm_term(numbers: Tensor, positions: Tensor, param: dict, cn: Tensor,
 
 def dftd4(numbers: Tensor, positions: Tensor, charge, param: dict,
           q: Tensor | None = None, cutoff: dict | None = None,
-          return_parts: bool = False, nan_safe: bool = False):
+          return_parts: bool = False, nan_safe: bool = False, model: str = "d4"):
     """``dftd4`` (``disp.py:44-146``) -> ``Disp.calculate``
     (``dispersion/base.py:285-431``): CN, EEQ charges, two-body term with
-    charge-dependent C6 and the approximate-C9 ATM term.
+    charge-dependent C6 and the approximate-C9 ATM term; ``model`` "d4" or
+    "d4s" (pairwise Gaussian weighting factors).
 
     ``cutoff`` keys: ``disp2``, ``disp3``, ``cn_eeq`` (``cutoff.py:36-90``;
     ``cn`` is accepted and ignored, as in the reference)."""
@@ -554,23 +666,24 @@ def dftd4(numbers: Tensor, positions: Tensor, charge, param: dict,
     if q is None:
         q = eeq_charges(numbers, positions, charge, cutoff=c["cn_eeq"])
     rc6 = get_refc6(numbers, dtype)
-    gw = weight_references(numbers, cn, q, dtype=dtype, nan_safe=nan_safe)
-    c6 = get_atomic_c6(rc6, gw)
+    wref, c6fn = _model_fns(model)
+    gw = wref(numbers, cn, q, dtype=dtype, nan_safe=nan_safe)
+    c6 = c6fn(rc6, gw)
     e2 = dispersion2(numbers, positions, param, c6, r4r2, cutoff=c["disp2"])
-    e3 = atm_term(numbers, positions, param, cn, rc6, r4r2, c["disp3"], nan_safe)
+    e3 = atm_term(numbers, positions, param, cn, rc6, r4r2, c["disp3"], nan_safe, model)
     if return_parts:
         return {"e2": e2, "e3": e3, "cn": cn, "q": q, "c6": c6, "gw": gw}
     return e2 + e3
 
 
 def dftd4_with_grad(numbers: Tensor, positions: Tensor, charge, param: dict,
-                    cutoff: dict | None = None, nan_safe: bool = True
+                    cutoff: dict | None = None, nan_safe: bool = True, model: str = "d4"
                     ) -> tuple[Tensor, Tensor]:
     """Atom energies and the autograd gradient dE/dR of their sum
     (the reference's force path, ``examples/forces.py:47-50``). By default
     the exceptional-weight branch is made NaN-safe (see
     ``weight_references``); energies are identical either way."""
     pos = positions.detach().clone().requires_grad_(True)
-    e = dftd4(numbers, pos, charge, param, cutoff=cutoff, nan_safe=nan_safe)
+    e = dftd4(numbers, pos, charge, param, cutoff=cutoff, nan_safe=nan_safe, model=model)
     (g,) = torch.autograd.grad(e.sum(), pos)
     return e.detach(), g

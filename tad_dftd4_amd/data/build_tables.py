@@ -16,6 +16,8 @@ Sources (paths relative to the reference checkout):
 - ``src/tad_dftd4/reference/d4/charge_eeq.py``: ``clsq`` (:29), ``clsh`` (:971)
 - ``src/tad_dftd4/data/r4r2.py``: ``_r4_over_r2`` (:55-85)
 - ``src/tad_dftd4/utils.py``: Casimir-Polder weights of ``trapzd`` (:52-80)
+- ``src/tad_dftd4/data/wfpair.py``: ``_wfpair`` (:40), the pairwise Gaussian
+  weighting factors of the D4S model, 119 x 119
 
 Usage::
 
@@ -89,6 +91,7 @@ def build(reference_root: str | Path) -> dict[str, np.ndarray]:
     charge = (root / "reference" / "d4" / "charge_eeq.py").read_text()
     r4r2 = (root / "data" / "r4r2.py").read_text()
     utils = (root / "utils.py").read_text()
+    wfpair = (root / "data" / "wfpair.py").read_text()
 
     t: dict[str, np.ndarray] = {}
     for name in ("refcovcn", "refalpha", "refascale", "refscount"):
@@ -103,6 +106,7 @@ def build(reference_root: str | Path) -> dict[str, np.ndarray]:
         t[name] = np.asarray(_tensor_literal(charge, name), dtype=np.float64)
     t["r4_over_r2"] = np.asarray(_list_literal(r4r2, "_r4_over_r2"), np.float64)
     t["cp_weights"] = np.asarray(_trapzd_weights(utils), dtype=np.float64)
+    t["wfpair"] = np.asarray(_list_literal(wfpair, "_wfpair"), np.float64).reshape(119, 119)
 
     expect = {
         "refcovcn": (104, 7),
@@ -117,6 +121,7 @@ def build(reference_root: str | Path) -> dict[str, np.ndarray]:
         "clsh": (104, 7),
         "r4_over_r2": (119,),
         "cp_weights": (23,),
+        "wfpair": (119, 119),
     }
     for k, shape in expect.items():
         if t[k].shape != shape:

@@ -15,7 +15,10 @@ Sources (relative to the reference checkout):
 - ``test/test_grad/samples_grad.py`` LiH and SiH4 ``grad`` (tblite, TPSS0
   parameters of ``test/test_grad/test_pos.py:55-62``).
 - ``test/test_model/samples.py`` LiH and SiH4 ``gw`` (Fortran-ordered
-  ``(nref, nat)``) and ``c6``.
+  ``(nref, nat)``) and ``c6``; the D4S ``gw_d4s`` (Fortran-ordered
+  ``(nref, nat, nat)``) and ``c6_d4s``.
+- ``test/test_d4/samples.py`` LiH and SiH4 ``disp2_d4s`` (two-body energy
+  with the D4S model).
 
 The LiH and SiH4 geometries live in tad-mctc (absent); they are restated
 below from tad-mctc's molecule set (``data/molecules.py``: LiH bond along z,
@@ -154,6 +157,9 @@ def build(ref: Path) -> dict:
             "grad": _entry(g, "grad"),
             "gw": _entry(m, "gw"),
             "c6": _entry(m, "c6"),
+            "disp2_d4s": _entry(b, "disp2_d4s"),
+            "gw_d4s": _entry(m, "gw_d4s"),
+            "c6_d4s": _entry(m, "c6_d4s"),
         }
     out["param_tpssh"] = {
         "s6": 1.0, "s8": 1.85897750, "s9": 1.0, "s10": 0.0, "alp": 16.0,

@@ -135,6 +135,103 @@ def test_weights_fixture_and_normalisation(name):
     assert torch.allclose(s, torch.ones_like(s), atol=1e-14)
 
 
+# ------------------------------------------------------------------ D4S
+
+
+def _fixture_gw_d4s(name):
+    """gw_d4s is Fortran-ordered (nref, nat, nat); test/test_model/test_c6.py:
+    63-82 pads it to 7 references and permutes it to (nat, nat, ref)."""
+    g = GOLD[name]["gw_d4s"]
+    nref, na, nb = g["fortran_shape"]
+    src = torch.tensor(g["values"], dtype=F64).reshape(nb, na, nref).permute(2, 1, 0)
+    src = torch.nn.functional.pad(src, (0, 0, 0, 0, 0, 7 - nref))
+    return src.permute(2, 1, 0)
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_d4s_two_body_fixture(name):
+    """test/test_d4/samples.py disp2_d4s (dftd4 Fortran D4S two-body energy),
+    same call as test/test_d4/test_twobody.py:68-108 with the fixture
+    charges (reference tolerance 1.5e-7); the D4 fixture matches to 1e-10."""
+    numbers, positions = _mol(name)
+    q = torch.tensor(GOLD[name]["q"], dtype=F64)
+    parts = dense.dftd4(numbers, positions, 0.0, _p(GOLD["param_tpssh"]), q=q,
+                        return_parts=True, model="d4s")
+    ref = torch.tensor(GOLD[name]["disp2_d4s"], dtype=F64)
+    assert (parts["e2"] - ref).abs().max().item() < 1e-9
+    # the D4 model is measurably different on the same input
+    d4 = dense.dftd4(numbers, positions, 0.0, _p(GOLD["param_tpssh"]), q=q,
+                     return_parts=True)
+    assert (d4["e2"] - ref).abs().max().item() > 1e-6
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_d4s_c6_and_weights_fixture(name):
+    """c6_d4s from the fixture weights (test/test_model/test_c6.py:63-82):
+    pins the partner/atom index order of the D4S contraction; weights from
+    the fixture geometry and charges agree with gw_d4s to the D4 bound."""
+    numbers, positions = _mol(name)
+    n = len(numbers)
+    gw_ref = _fixture_gw_d4s(name)
+    c6 = dense.get_atomic_c6_d4s(dense.get_refc6(numbers), gw_ref)
+    ref = torch.tensor(GOLD[name]["c6_d4s"], dtype=F64).reshape(n, n)
+    assert torch.allclose(c6, ref, rtol=1e-10, atol=0)
+    q = torch.tensor(GOLD[name]["q"], dtype=F64)
+    gw = dense.weight_references_d4s(numbers, dense.cn_d4(numbers, positions), q)
+    assert torch.allclose(gw, gw_ref, rtol=1e-6, atol=1e-12)
+
+
+def test_d4s_reduces_to_d4_for_constant_factor(monkeypatch):
+    """With every pairwise factor equal to the D4 factor (6) the D4S weights
+    are the D4 weights for every partner, and the two models give the same
+    energies; with the real table the C6 matrix stays symmetric and each
+    partner's weights are normalised (sum_r gw / zeta = 1)."""
+    m = _rand_small(3, 14)
+    numbers, positions = torch.tensor(m.numbers), torch.tensor(m.positions)
+    t = dense.tables()
+    cn = dense.cn_d4(numbers, positions)
+    q = dense.eeq_charges(numbers, positions, torch.tensor(m.charge, dtype=F64))
+    gw4 = dense.weight_references(numbers, cn, q)
+    gws = dense.weight_references_d4s(numbers, cn, q)
+    rc6 = dense.get_refc6(numbers)
+    c6s = dense.get_atomic_c6_d4s(rc6, gws)
+    assert torch.allclose(c6s, c6s.mT, rtol=1e-14, atol=0)
+    refc = t.refc[numbers]
+    zeff = t.zeff[numbers].unsqueeze(-1)
+    z = dense.zeta(t.gam[numbers].unsqueeze(-1) * dense.GC,
+                   t.clsq[numbers] + zeff, q.unsqueeze(-1) + zeff)
+    s = torch.where(refc > 0, gws / z, torch.zeros_like(gws)).sum(-1)
+    assert torch.allclose(s, torch.ones_like(s), atol=1e-13)
+    assert (gws - gw4.unsqueeze(0)).abs().max().item() > 1e-3
+    const = dense.Tables.__new__(dense.Tables)
+    const.__dict__.update(t.__dict__)
+    const.wfpair = torch.full_like(t.wfpair, 6.0)
+    monkeypatch.setattr(dense, "tables", lambda dtype=F64: const)
+    gwc = dense.weight_references_d4s(numbers, cn, q)
+    assert torch.allclose(gwc, gw4.unsqueeze(0).expand_as(gwc), rtol=1e-12, atol=1e-300)
+    par = _p(GOLD["param_tpssh"])
+    e4 = dense.dftd4(numbers, positions, m.charge, par)
+    es = dense.dftd4(numbers, positions, m.charge, par, model="d4s")
+    assert torch.allclose(es, e4, rtol=1e-12, atol=1e-18)
+
+
+def test_d4s_gradient_finite_difference():
+    """Analytic (nan-safe autograd) D4S gradient vs central differences."""
+    m = _rand_small(5, 10)
+    numbers, positions = torch.tensor(m.numbers), torch.tensor(m.positions)
+    par = _p(GOLD["param_tpssh"])
+    _, g = dense.dftd4_with_grad(numbers, positions, m.charge, par, model="d4s")
+    h = 1e-5
+    for i, k in [(0, 0), (3, 1), (7, 2)]:
+        pp = positions.clone()
+        pm = positions.clone()
+        pp[i, k] += h
+        pm[i, k] -= h
+        fd = (dense.dftd4(numbers, pp, m.charge, par, model="d4s").sum()
+              - dense.dftd4(numbers, pm, m.charge, par, model="d4s").sum()) / (2 * h)
+        assert abs(fd.item() - g[i, k].item()) < 1e-9
+
+
 # ------------------------------------------------------------- brute force
 
 
