@@ -41,6 +41,8 @@ extern "C" {
 #define D4_FLAG_FORCE_TILED 4 /* testing: use the large-system (tiled) EEQ path at any size */
 #define D4_FLAG_FORCE_SORT 8  /* testing: spatially sort the atoms of every molecule */
 #define D4_FLAG_FORCE_PAIR_TILES 16 /* testing: three-body term by pair tiles at any size */
+#define D4_FLAG_D4S 32     /* D4S model: Gaussian weights with pairwise factors
+                              (reference src/tad_dftd4/model/d4s.py:56-285) */
 
 /* Element and reference-system tables, host memory, read once by
  * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference
@@ -63,6 +65,8 @@ typedef struct d4_tables {
   const double* refq;  /* reference charges */
   const double* alpha0; /* static reference polarisabilities alpha(i*0) */
   const double* rc6;   /* [zi][zj][a][b] reference C6, D4_MAX_Z^2 * 49 */
+  const double* wfpair; /* [zi][zj] D4S Gaussian weighting factors, D4_MAX_Z^2
+                           (src/tad_dftd4/data/wfpair.py); NULL: no D4S */
 } d4_tables;
 
 /* Damping parameters, model constants and cutoffs. Absent optional terms
@@ -137,7 +141,8 @@ int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch,
 /* Per-atom properties: D4 coordination numbers, charges, the C6 matrix
  * (nbatch, natoms, natoms) and static polarisabilities (nbatch, natoms).
  * Replaces get_properties() (reference src/tad_dftd4/disp.py:149-197).
- * Any output pointer may be NULL. */
+ * Any output pointer may be NULL. D4 model only (as the reference, which
+ * builds a D4Model there): D4_FLAG_D4S is rejected. */
 int d4_properties(d4_context* ctx, void* stream, int64_t nbatch,
                   int64_t natoms, const int64_t* numbers,
                   const double* positions, const double* charge,

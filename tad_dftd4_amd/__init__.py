@@ -15,13 +15,14 @@ from .__version__ import __version__
 from .cutoff import Cutoff
 from .damping import Damping, Param, RationalDamping, ZeroDamping
 from .disp import dftd4, dftd4_energy_gradient, get_properties
-from .model import D4Model
+from .model import D4Model, D4SModel
 from .ncoord import cn_d4, erf_count
 
 __all__ = [
     "__version__",
     "Cutoff",
     "D4Model",
+    "D4SModel",
     "Damping",
     "Param",
     "RationalDamping",

@@ -26,6 +26,7 @@ D4_FLAG_FORCE_LU = 2
 D4_FLAG_FORCE_TILED = 4
 D4_FLAG_FORCE_SORT = 8
 D4_FLAG_FORCE_PAIR_TILES = 16
+D4_FLAG_D4S = 32
 
 SLOTS = ("prep", "cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain")
 
@@ -55,6 +56,7 @@ class Tables(ctypes.Structure):
         ("refq", _dp),
         ("alpha0", _dp),
         ("rc6", _dp),
+        ("wfpair", _dp),
     ]
 
 

@@ -18,6 +18,7 @@ struct d4_context {
   int device;
   void* mem;
   d4::DevTables t;
+  bool has_wfpair = false;  // D4S factors supplied at creation
   // diagnostics: per-kernel event timing
   bool profile = false;
   std::vector<d4_event_pair> pending;
@@ -67,6 +68,7 @@ d4::Params to_params(const d4_params* p) {
   k.force_tiled = (p->flags & D4_FLAG_FORCE_TILED) ? 1 : 0;
   k.force_sort = (p->flags & D4_FLAG_FORCE_SORT) ? 1 : 0;
   k.force_pairtiles = (p->flags & D4_FLAG_FORCE_PAIR_TILES) ? 1 : 0;
+  k.d4s = (p->flags & D4_FLAG_D4S) ? 1 : 0;
   return k;
 }
 
@@ -90,6 +92,8 @@ int check_common(const d4_context* ctx, int64_t nbatch, int64_t natoms,
   if (!(params->cutoff_disp2 >= 0.0) || !(params->cutoff_disp3 >= 0.0) ||
       !(params->cutoff_cn >= 0.0) || !(params->cutoff_eeq >= 0.0))
     return set_error(D4_ERR_INVALID, "cutoffs must be non-negative");
+  if ((params->flags & D4_FLAG_D4S) && !ctx->has_wfpair)
+    return set_error(D4_ERR_INVALID, "D4_FLAG_D4S needs the wfpair table at context creation");
   return D4_OK;
 }
 
@@ -203,7 +207,8 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
 
   const size_t ne = D4_MAX_Z, nr = size_t(D4_MAX_Z) * D4_MAX_REF;
   const size_t nrc6 = size_t(D4_MAX_Z) * D4_MAX_Z * D4_MAX_REF * D4_MAX_REF;
-  const size_t ndbl = 10 * ne + 3 * nr + nrc6;
+  const size_t nwf = size_t(D4_MAX_Z) * D4_MAX_Z;
+  const size_t ndbl = 10 * ne + 3 * nr + nrc6 + nwf;
   const size_t nint = ne + nr;
   std::vector<double> hd;
   hd.reserve(ndbl);
@@ -212,6 +217,10 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   hd.insert(hd.end(), tb->refq, tb->refq + nr);
   hd.insert(hd.end(), tb->alpha0, tb->alpha0 + nr);
   hd.insert(hd.end(), tb->rc6, tb->rc6 + nrc6);
+  if (tb->wfpair)
+    hd.insert(hd.end(), tb->wfpair, tb->wfpair + nwf);
+  else
+    hd.insert(hd.end(), nwf, 0.0);
   std::vector<int32_t> hi;
   hi.insert(hi.end(), tb->nref, tb->nref + ne);
   hi.insert(hi.end(), tb->ngw, tb->ngw + nr);
@@ -241,6 +250,7 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   d4_context* ctx = new d4_context;
   ctx->device = device;
   ctx->mem = mem;
+  ctx->has_wfpair = tb->wfpair != nullptr;
   const double* d = (const double*)mem;
   d4::DevTables& t = ctx->t;
   t.rcov = d + 0 * ne;
@@ -257,6 +267,7 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   t.refq = t.refcn + nr;
   t.alpha0 = t.refq + nr;
   t.rc6 = t.alpha0 + nr;
+  t.wfpair = t.rc6 + nrc6;
   const int32_t* ip = (const int32_t*)(d + ndbl);
   t.nref = ip;
   t.ngw = ip + ne;
@@ -372,7 +383,10 @@ int d4_properties(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
                   double* alpha_out, void* workspace, size_t workspace_bytes) {
   int rc = check_common(ctx, nbatch, natoms, numbers, positions, charge, params,
                         workspace, workspace_bytes);
-  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (rc != D4_OK) return rc;
+  if (params->flags & D4_FLAG_D4S)
+    return set_error(D4_ERR_INVALID, "d4_properties evaluates the D4 model only");
+  if (nbatch == 0 || natoms == 0) return rc;
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, nullptr,
                      nullptr, params, workspace);
   if ((rc = run_front(ctx, c)) != D4_OK) return rc;

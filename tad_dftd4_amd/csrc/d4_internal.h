@@ -35,6 +35,7 @@ struct DevTables {
   const double* refq;
   const double* alpha0;
   const double* rc6;
+  const double* wfpair;  // D4S pairwise weighting factors [zi][zj]
   const int32_t* nref;
   const int32_t* ngw;
 };
@@ -50,6 +51,7 @@ struct Params {
   int force_sort;
   int force_pairtiles;
   int pgrad;  // accumulate parameter gradients (backward only)
+  int d4s;    // D4S model: Gaussian weights with pairwise factors
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).
@@ -65,11 +67,11 @@ struct Work {
   double* dldq;     // [B*N] dL/dq
   double* beta;     // [B*N] lambda_i * d b_i / d raw EEQ CN
   double* lam;      // [B*N] EEQ adjoint
-  double* gw;       // [B*N*8] weights with charges
-  double* gw0;      // [B*N*8] weights with q = 0 (ATM)
-  double* dgw;      // [B*N*8] d gw / d CN
-  double* dgw0;     // [B*N*8] d gw0 / d CN
-  double* dgwq;     // [B*N*8] d gw / d q
+  double* gw;       // [B*N*8] weights with charges (D4S: zeta(q))
+  double* gw0;      // [B*N*8] weights with q = 0 (ATM) (D4S: zeta(0))
+  double* dgw;      // [B*N*8] d gw / d CN (D4S: unused)
+  double* dgw0;     // [B*N*8] d gw0 / d CN (D4S: unused)
+  double* dgwq;     // [B*N*8] d gw / d q (D4S: dzeta/dq)
   double* chol;     // [B*NT] packed lower Cholesky factor of the Coulomb block
   double* ez;       // [B*N] y_e = L^-1 e (forward-substituted border, e = 1 on real atoms)
   double* lu;       // [B*(N+1)^2] pivoted LU of the bordered matrix (fallback)

@@ -118,6 +118,29 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
   const int nr = z != 0 ? t.nref[z] : 0;
   const double cn = z != 0 ? w.cn[row] : 0.0;
   const double q = z != 0 ? w.q[row] : 0.0;
+  if (p.d4s) {
+    // D4S: the Gaussian part depends on the partner atom and is formed in
+    // the pair pass; keep the charge scaling zeta(q), zeta(0), dzeta/dq of
+    // every reference (model/d4s.py:240-263)
+    const double zeff = t.zeff[z];
+    const double gam = t.gam[z] * p.gc;
+#pragma unroll
+    for (int a = 0; a < kRefStride; ++a) {
+      double zq = 0.0, z0 = 0.0, dz = 0.0;
+      if (a < nr) {
+        const double qref = t.refq[z * kMaxRef + a] + zeff;
+        zq = zeta_fn(p.ga, gam, qref, q + zeff);
+        z0 = zeta_fn(p.ga, gam, qref, zeff);
+        dz = dzeta_fn(p.ga, gam, qref, q + zeff);
+      }
+      gw[a] = zq;
+      gw0[a] = z0;
+      dgw[a] = 0.0;
+      dgw0[a] = 0.0;
+      dgwq[a] = dz;
+    }
+    return;
+  }
   double expw[kMaxRef], dexpw[kMaxRef];
   double norm = 0.0, dnorm = 0.0;
 #pragma unroll
@@ -178,12 +201,64 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
 }
 
 // --------------------------------------------------------------------------
-// pair pass: C6(q) and C6(0) contraction with the reference-C6 table, the
-// rational-damped two-body term, the ATM pair factors; with GRAD also the
-// two-body gradient pieces (one wave per atom row)
+// D4S Gaussian weights (model/d4s.py:178-238): normalised weights of an atom
+// of element z with coordination number cn for the pairwise factor wf of one
+// partner, exp(-wf k (cn - cn_ref)^2) summed over k = 1..ngw, and with DCN
+// their derivatives with respect to cn. When every Gaussian underflows the
+// weight goes to the reference(s) with the largest reference CN.
 // --------------------------------------------------------------------------
 
-template <bool GRAD, bool PG, int RL>
+template <bool DCN>
+__device__ __forceinline__ void pair_gauss(const DevTables& t, int z, int nr, double cn,
+                                           double wf, double* g, double* dg) {
+  double e[kMaxRef], de[kMaxRef];
+  double norm = 0.0, dnorm = 0.0;
+#pragma unroll
+  for (int a = 0; a < kMaxRef; ++a) {
+    e[a] = 0.0;
+    de[a] = 0.0;
+    if (a < nr) {
+      const double d = cn - t.refcn[z * kMaxRef + a];
+      const double tmp = exp(-(d * d) * wf);
+      double sum = tmp, ksum = tmp;  // sum_k tmp^k and sum_k k tmp^k
+      if (t.ngw[z * kMaxRef + a] == 3) {
+        const double t2 = tmp * tmp, t3 = t2 * tmp;
+        sum = tmp + t2 + t3;
+        ksum = fma(3.0, t3, fma(2.0, t2, tmp));
+      }
+      e[a] = sum;
+      if (DCN) de[a] = -2.0 * wf * d * ksum;
+      norm += sum;
+      if (DCN) dnorm += de[a];
+    }
+  }
+  const bool exceptional = !(norm > 0.0);
+  const double maxcn = t.maxcn[z];
+#pragma unroll
+  for (int a = 0; a < kMaxRef; ++a) {
+    double v = 0.0, dv = 0.0;
+    if (a < nr) {
+      if (exceptional) {
+        v = t.refcn[z * kMaxRef + a] == maxcn ? 1.0 : 0.0;
+      } else {
+        v = e[a] / norm;
+        if (DCN) dv = (de[a] - v * dnorm) / norm;
+      }
+    }
+    g[a] = v;
+    if (DCN) dg[a] = dv;
+  }
+}
+
+// --------------------------------------------------------------------------
+// pair pass: C6(q) and C6(0) contraction with the reference-C6 table, the
+// rational-damped two-body term, the ATM pair factors; with GRAD also the
+// two-body gradient pieces (one wave per atom row). With S (D4S model) the
+// weights of both atoms are formed per pair from the charge scaling that
+// k_weights left and the pair's Gaussian weights.
+// --------------------------------------------------------------------------
+
+template <bool GRAD, bool PG, bool S, int RL>
 __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
                                                Work w, double* grad_out) {
   const int lane = threadIdx.x & (RL - 1);
@@ -235,6 +310,7 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
       dgwqi[a] = w.dgwq[row * kRefStride + a];
     }
   }
+  const double cni = S ? w.cn[row] : 0.0;
   const double gi = GRAD ? (in.gE ? in.gE[row] : 1.0) : 0.0;
   const double alp3 = p.alp / 3.0;
   double acc_e = 0.0, acc_cn = 0.0, acc_q = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
@@ -259,6 +335,37 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
       gwj[b] = w.gw[rj * kRefStride + b];
       gw0j[b] = w.gw0[rj * kRefStride + b];
     }
+    // this pair's weights of atom i (D4: the row's own weights)
+    double pgw[kMaxRef], pgw0[kMaxRef], pdgw[kMaxRef], pdgw0[kMaxRef], pdgwq[kMaxRef];
+    if (S) {
+      // atom n's weights for partner m use wf(Z_n, Z_m) (model/d4s.py:196)
+      double g[kMaxRef], dg[kMaxRef], gj[kMaxRef];
+      pair_gauss<GRAD>(t, zi, nri, cni, t.wfpair[zi * kMaxZ + zj], g, dg);
+      pair_gauss<false>(t, zj, t.nref[zj], w.cn[rj], t.wfpair[zj * kMaxZ + zi], gj, nullptr);
+#pragma unroll
+      for (int a = 0; a < kMaxRef; ++a) {
+        pgw[a] = gwi[a] * g[a];
+        pgw0[a] = gw0i[a] * g[a];
+        if (GRAD) {
+          pdgw[a] = gwi[a] * dg[a];
+          pdgw0[a] = gw0i[a] * dg[a];
+          pdgwq[a] = dgwqi[a] * g[a];
+        }
+        gwj[a] *= gj[a];
+        gw0j[a] *= gj[a];
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < kMaxRef; ++a) {
+        pgw[a] = gwi[a];
+        pgw0[a] = gw0i[a];
+        if (GRAD) {
+          pdgw[a] = dgwi[a];
+          pdgw0[a] = dgw0i[a];
+          pdgwq[a] = dgwqi[a];
+        }
+      }
+    }
     const double* blk = t.rc6 + ((size_t)zi * kMaxZ + zj) * kRc6Block;
     double c6 = 0.0, c60 = 0.0, dc6cn = 0.0, dc6q = 0.0, dc60cn = 0.0;
     for (int a = 0; a < nri; ++a) {
@@ -269,12 +376,12 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
         va += rc * gwj[b];
         v0a += rc * gw0j[b];
       }
-      c6 += gwi[a] * va;
-      c60 += gw0i[a] * v0a;
+      c6 += pgw[a] * va;
+      c60 += pgw0[a] * v0a;
       if (GRAD) {
-        dc6cn += dgwi[a] * va;
-        dc6q += dgwqi[a] * va;
-        dc60cn += dgw0i[a] * v0a;
+        dc6cn += pdgw[a] * va;
+        dc6q += pdgwq[a] * va;
+        dc60cn += pdgw0[a] * v0a;
       }
     }
     // rational damping, radii = 3 r4r2_i r4r2_j (twobody.py:147-157)
@@ -1213,18 +1320,27 @@ hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
   return hipGetLastError();
 }
 
+template <bool S>
+void launch_pairs_model(hipStream_t s, const DevTables& t, const Params& p, const Inputs& in,
+                        const Work& w, bool grad, double* grad_out) {
+  if (grad && p.pgrad)
+    D4_LAUNCH_ROWS((k_pairs<true, true, S, 16>), (k_pairs<true, true, S, 32>),
+                   (k_pairs<true, true, S, 64>), s, w, t, p, in, w, grad_out);
+  else if (grad)
+    D4_LAUNCH_ROWS((k_pairs<true, false, S, 16>), (k_pairs<true, false, S, 32>),
+                   (k_pairs<true, false, S, 64>), s, w, t, p, in, w, grad_out);
+  else
+    D4_LAUNCH_ROWS((k_pairs<false, false, S, 16>), (k_pairs<false, false, S, 32>),
+                   (k_pairs<false, false, S, 64>), s, w, t, p, in, w, grad_out);
+}
+
 hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, bool grad,
                         double* grad_out) {
-  if (grad && p.pgrad)
-    D4_LAUNCH_ROWS((k_pairs<true, true, 16>), (k_pairs<true, true, 32>),
-                   (k_pairs<true, true, 64>), s, w, t, p, in, w, grad_out);
-  else if (grad)
-    D4_LAUNCH_ROWS((k_pairs<true, false, 16>), (k_pairs<true, false, 32>),
-                   (k_pairs<true, false, 64>), s, w, t, p, in, w, grad_out);
+  if (p.d4s)
+    launch_pairs_model<true>(s, t, p, in, w, grad, grad_out);
   else
-    D4_LAUNCH_ROWS((k_pairs<false, false, 16>), (k_pairs<false, false, 32>),
-                   (k_pairs<false, false, 64>), s, w, t, p, in, w, grad_out);
+    launch_pairs_model<false>(s, t, p, in, w, grad, grad_out);
   return hipGetLastError();
 }
 

@@ -12,7 +12,7 @@ from hand-written backward kernels through ``torch.autograd``.
 
 Not supported (raise ``NotImplementedError``; there is no fallback):
 non-default ``cn_function``/``counting_function``/``damping_function``,
-models other than D4 with EEQ reference charges, gradients with respect to
+models other than D4 and D4S with EEQ reference charges, gradients with respect to
 user-supplied charges, double backward. Gradients with respect to the damping
 parameters s6, s8, s10, s9, a1, a2 and alp (0-d tensors with
 ``requires_grad``) come from the same backward kernels
@@ -26,7 +26,7 @@ import torch
 from . import defaults, runtime
 from .cutoff import Cutoff
 from .damping import Damping, Param, RationalDamping
-from .model import D4Model
+from .model import D4Model, D4SModel
 from .ncoord import cn_d4, erf_count
 
 __all__ = ["dftd4", "get_properties", "dftd4_energy_gradient"]
@@ -80,6 +80,7 @@ def kernel_params(param: Param, cutoff: Cutoff | None, model: D4Model | None):
     )
     if model is not None:
         kp.ga, kp.gc, kp.wf = model.ga, model.gc, model.wf
+        kp.d4s = isinstance(model, D4SModel)
     return kp
 
 
@@ -126,9 +127,9 @@ def _prepare(numbers, positions, charge, model, rcov, r4r2, rvdw, q, cn_function
         key = model.casefold()
         if key not in _MODELS:
             raise ValueError(f"Unknown model '{key}'. Please use {', '.join(_MODELS)}.")
-        if key != "d4":
+        if key not in ("d4", "d4s"):
             raise NotImplementedError(f"model '{key}' is not implemented by the MI355X kernels")
-        model_inst = None
+        model_inst = D4SModel(numbers) if key == "d4s" else None
     elif isinstance(model, D4Model):
         model_inst = model
     else:

@@ -1,6 +1,7 @@
 """
-``D4Model``: the D4 dispersion model's constants (mirrors reference
-``src/tad_dftd4/model/d4.py`` / ``model/base.py:107-151``).
+``D4Model`` and ``D4SModel``: the D4 and D4S dispersion models' constants
+(mirror reference ``src/tad_dftd4/model/d4.py``, ``model/d4s.py`` and
+``model/base.py:107-151``).
 
 The reference rebuilds per-atom reference polarisabilities and the
 per-atom-pair reference C6 tensor on every call. Here the model only carries
@@ -17,7 +18,7 @@ import torch
 
 from .tables import GA_DEFAULT, GC_DEFAULT, MAX_REF, WF_DEFAULT, device_tables
 
-__all__ = ["D4Model"]
+__all__ = ["D4Model", "D4SModel"]
 
 
 class D4Model:
@@ -74,4 +75,34 @@ class D4Model:
         return self._rc6
 
     def __repr__(self) -> str:
-        return f"D4Model(ga={self.ga}, gc={self.gc}, wf={self.wf}, ref_charges={self.ref_charges!r})"
+        return (f"{type(self).__name__}(ga={self.ga}, gc={self.gc}, wf={self.wf}, "
+                f"ref_charges={self.ref_charges!r})")
+
+
+class D4SModel(D4Model):
+    """The D4S dispersion model (``model/d4s.py:56-285``): the Gaussian
+    weights of atom A in the pair AB use the element-pair factor
+    wf(Z_A, Z_B) of ``data/wfpair.py`` instead of one global factor, so every
+    pair has its own weights (and C6). The factors are resident on the device
+    with the other tables; the kernels form the pair weights in the pair
+    pass."""
+
+    __slots__ = ()
+
+    def __init__(
+        self,
+        numbers: torch.Tensor,
+        ga: float = GA_DEFAULT,
+        gc: float = GC_DEFAULT,
+        wf: torch.Tensor | float | None = None,
+        ref_charges: str = "eeq",
+        rc6: torch.Tensor | None = None,
+        device: torch.device | None = None,
+        dtype: torch.dtype | None = None,
+    ) -> None:
+        if wf is not None:
+            raise NotImplementedError(
+                "D4SModel with user-supplied weighting factors is not supported: "
+                "the kernels read the element-pair factor table resident on the device"
+            )
+        super().__init__(numbers, ga, gc, None, ref_charges, rc6, device, dtype)

@@ -36,7 +36,8 @@ def context(device: torch.device, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT
             return _CTX[key]
         t = device_tables(float(ga), float(gc))
         arrays = [t.rcov, t.en, t.r4r2, t.eeq_chi, t.eeq_eta, t.eeq_kcn, t.eeq_rad,
-                  t.gam, t.zeff, t.maxcn, t.refcn, t.refq, t.alpha0, t.rc6, t.nref, t.ngw]
+                  t.gam, t.zeff, t.maxcn, t.refcn, t.refq, t.alpha0, t.rc6, t.nref, t.ngw,
+                  t.wfpair]
         _KEEPALIVE.append(arrays)
         tb = _lib.Tables(
             rcov=_ptr(t.rcov, ctypes.c_double),
@@ -55,6 +56,7 @@ def context(device: torch.device, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT
             refq=_ptr(t.refq, ctypes.c_double),
             alpha0=_ptr(t.alpha0, ctypes.c_double),
             rc6=_ptr(t.rc6, ctypes.c_double),
+            wfpair=_ptr(t.wfpair, ctypes.c_double),
         )
         out = ctypes.c_void_p()
         _lib.check(lib.d4_context_create(index, ctypes.byref(tb), ctypes.byref(out)),
@@ -97,6 +99,7 @@ class KernelParams:
     force_tiled: bool = False
     force_sort: bool = False
     force_pair_tiles: bool = False
+    d4s: bool = False  # D4S model (pairwise Gaussian weighting factors)
 
     def c(self) -> _lib.Params:
         flags = (
@@ -105,6 +108,7 @@ class KernelParams:
             | (_lib.D4_FLAG_FORCE_TILED if self.force_tiled else 0)
             | (_lib.D4_FLAG_FORCE_SORT if self.force_sort else 0)
             | (_lib.D4_FLAG_FORCE_PAIR_TILES if self.force_pair_tiles else 0)
+            | (_lib.D4_FLAG_D4S if self.d4s else 0)
         )
         return _lib.Params(
             self.s6, self.s8, self.s10, self.s9, self.a1, self.a2, self.alp,

@@ -75,6 +75,7 @@ class DeviceTables:
     refq: np.ndarray
     alpha0: np.ndarray
     rc6: np.ndarray
+    wfpair: np.ndarray  # D4S pairwise Gaussian weighting factors (Z, Z)
     perm: np.ndarray  # compacted slot -> original slot (-1: unused)
 
 
@@ -127,5 +128,6 @@ def device_tables(ga: float = GA_DEFAULT, gc: float = GC_DEFAULT) -> DeviceTable
         refq=c(compact(t["clsq"])),
         alpha0=c(alpha[..., 0]),
         rc6=c(rc6),
+        wfpair=c(t["wfpair"][:MAX_Z, :MAX_Z]),
         perm=perm,
     )

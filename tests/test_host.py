@@ -103,7 +103,9 @@ def test_boundary_errors_before_device_work():
     with pytest.raises(ValueError):
         d4.dftd4(numbers, positions, 0.0, par, model="d9")
     with pytest.raises(NotImplementedError):
-        d4.dftd4(numbers, positions, 0.0, par, model="d4s")
+        d4.dftd4(numbers, positions, 0.0, par, model="d3")
+    with pytest.raises(NotImplementedError):
+        d4.dftd4(numbers, positions, 0.0, par, model=d4.D4SModel(numbers, wf=4.0))
     with pytest.raises(ValueError):
         d4.dftd4(numbers, positions, 0.0, par, r4r2=torch.ones(3))
     with pytest.raises(ValueError):
@@ -125,3 +127,10 @@ def test_cutoff_and_model_objects():
     assert torch.allclose(rc6.double(), ref.double(), rtol=1e-6)
     with pytest.raises(NotImplementedError):
         d4.D4Model(torch.tensor([6]), ref_charges="gfn2")
+    s = d4.D4SModel(torch.tensor([6, 1]))
+    assert isinstance(s, d4.D4Model) and repr(s).startswith("D4SModel(")
+    kp = d4.disp.kernel_params({"a1": 0.4, "a2": 5.0}, None, s)
+    assert kp.d4s and kp.c().flags & 32
+    assert not d4.disp.kernel_params({"a1": 0.4, "a2": 5.0}, None, m).d4s
+    t = d4.tables.device_tables()
+    assert t.wfpair.shape == (104, 104) and t.wfpair[6, 1] == t.wfpair[1, 6]
