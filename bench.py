@@ -22,7 +22,10 @@ Other workloads (``--workload``), one JSON line each:
                    continuous-batching loop at 70 % of its measured capacity
                    with Poisson arrivals: p50 / p99 latency.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+``--model d4s`` runs the batch workloads with the D4S model (element-pair
+Gaussian weighting factors) instead of D4.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--model d4|d4s]
 """
 
 from __future__ import annotations
@@ -55,6 +58,7 @@ def parse() -> argparse.Namespace:
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--requests", type=int, default=10000, help="serving_stream requests")
+    ap.add_argument("--model", choices=("d4", "d4s"), default="d4")
     return ap.parse_args()
 
 
@@ -105,7 +109,8 @@ def plain_gpu(mols, device, budget_s: float, workload: str = WORKLOAD) -> dict:
             "the GPU"}
 
 
-def cpu_baseline(mols, budget_s: float, workload: str = WORKLOAD, forces: bool = True) -> dict:
+def cpu_baseline(mols, budget_s: float, workload: str = WORKLOAD, forces: bool = True,
+                 model: str = "d4") -> dict:
     import torch
 
     from oracle import dense
@@ -117,10 +122,10 @@ def cpu_baseline(mols, budget_s: float, workload: str = WORKLOAD, forces: bool =
     for m in mols:
         if forces:
             dense.dftd4_with_grad(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge,
-                                  par, cutoff=cut)
+                                  par, cutoff=cut, model=model)
         else:
             dense.dftd4(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge, par,
-                        cutoff=cut)
+                        cutoff=cut, model=model)
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
@@ -196,6 +201,7 @@ def main() -> int:
     from tad_dftd4_amd import metrics, runtime, synth
     from tad_dftd4_amd.cutoff import Cutoff
     from tad_dftd4_amd.disp import kernel_params
+    from tad_dftd4_amd.model import D4SModel
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -206,7 +212,8 @@ def main() -> int:
     mols = synth.workload(args.workload)
     numbers, positions, charge = synth.pack(mols, device=dev)
     B, N = numbers.shape
-    kp = kernel_params(PARAM, Cutoff(disp2=CUT[0], disp3=CUT[1], cn_eeq=CUT[2]), None)
+    model = D4SModel(numbers) if args.model == "d4s" else None
+    kp = kernel_params(PARAM, Cutoff(disp2=CUT[0], disp3=CUT[1], cn_eeq=CUT[2]), model)
     ws = runtime.new_workspace(B, N, dev)
     energy = torch.empty((B, N), dtype=torch.float64, device=dev)
     grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
@@ -258,6 +265,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": args.workload,
+            "model": args.model,
             "molecules": B,
             "atoms_min_max": [int(min(len(m.numbers) for m in mols)),
                               int(max(len(m.numbers) for m in mols))],
@@ -295,11 +303,12 @@ def main() -> int:
                                "kind": "port", "sample": "not run: the dense restatement needs "
                                "n^3 = 1.25e11-element tensors (1 TB each) for 5000 atoms"}
     elif not args.no_baselines:
-        if forces:
+        if forces and args.model == "d4":
             pl = plain_gpu(mols, dev, args.plain_seconds, args.workload)
             out["plain_gpu"] = pl
             out["speedup_vs_plain"] = round(value / pl["value"], 1)
-        out["cpu_baseline"] = cpu_baseline(mols, args.cpu_seconds, args.workload, forces)
+        out["cpu_baseline"] = cpu_baseline(mols, args.cpu_seconds, args.workload, forces,
+                                           args.model)
     print(json.dumps(out), flush=True)
     return 0
 

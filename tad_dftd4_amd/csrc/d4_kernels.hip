@@ -233,16 +233,23 @@ __device__ __forceinline__ void pair_gauss(const DevTables& t, int z, int nr, do
     }
   }
   const bool exceptional = !(norm > 0.0);
+  // one reciprocal per atom, unless the norm is so small that 1/norm
+  // overflows (Gaussians near underflow): then divide per reference
+  const bool tiny = !(norm >= 0x1p-1000);
   const double maxcn = t.maxcn[z];
+  const double rnorm = tiny ? 0.0 : 1.0 / norm;
 #pragma unroll
   for (int a = 0; a < kMaxRef; ++a) {
     double v = 0.0, dv = 0.0;
     if (a < nr) {
       if (exceptional) {
         v = t.refcn[z * kMaxRef + a] == maxcn ? 1.0 : 0.0;
-      } else {
+      } else if (tiny) {
         v = e[a] / norm;
         if (DCN) dv = (de[a] - v * dnorm) / norm;
+      } else {
+        v = e[a] * rnorm;
+        if (DCN) dv = (de[a] - v * dnorm) * rnorm;
       }
     }
     g[a] = v;
@@ -305,8 +312,8 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
     gwi[a] = w.gw[row * kRefStride + a];
     gw0i[a] = w.gw0[row * kRefStride + a];
     if (GRAD) {
-      dgwi[a] = w.dgw[row * kRefStride + a];
-      dgw0i[a] = w.dgw0[row * kRefStride + a];
+      dgwi[a] = S ? 0.0 : w.dgw[row * kRefStride + a];
+      dgw0i[a] = S ? 0.0 : w.dgw0[row * kRefStride + a];
       dgwqi[a] = w.dgwq[row * kRefStride + a];
     }
   }
@@ -335,35 +342,18 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
       gwj[b] = w.gw[rj * kRefStride + b];
       gw0j[b] = w.gw0[rj * kRefStride + b];
     }
-    // this pair's weights of atom i (D4: the row's own weights)
-    double pgw[kMaxRef], pgw0[kMaxRef], pdgw[kMaxRef], pdgw0[kMaxRef], pdgwq[kMaxRef];
+    // D4S: Gaussian weights of both atoms for this pair; atom n's weights
+    // for partner m use wf(Z_n, Z_m) (model/d4s.py:196). The weights of
+    // atom i are zeta_i[a] g[a] (row arrays gwi, gw0i, dgwqi hold zeta).
+    double g[kMaxRef], dg[kMaxRef];
     if (S) {
-      // atom n's weights for partner m use wf(Z_n, Z_m) (model/d4s.py:196)
-      double g[kMaxRef], dg[kMaxRef], gj[kMaxRef];
       pair_gauss<GRAD>(t, zi, nri, cni, t.wfpair[zi * kMaxZ + zj], g, dg);
+      double gj[kMaxRef];
       pair_gauss<false>(t, zj, t.nref[zj], w.cn[rj], t.wfpair[zj * kMaxZ + zi], gj, nullptr);
 #pragma unroll
-      for (int a = 0; a < kMaxRef; ++a) {
-        pgw[a] = gwi[a] * g[a];
-        pgw0[a] = gw0i[a] * g[a];
-        if (GRAD) {
-          pdgw[a] = gwi[a] * dg[a];
-          pdgw0[a] = gw0i[a] * dg[a];
-          pdgwq[a] = dgwqi[a] * g[a];
-        }
-        gwj[a] *= gj[a];
-        gw0j[a] *= gj[a];
-      }
-    } else {
-#pragma unroll
-      for (int a = 0; a < kMaxRef; ++a) {
-        pgw[a] = gwi[a];
-        pgw0[a] = gw0i[a];
-        if (GRAD) {
-          pdgw[a] = dgwi[a];
-          pdgw0[a] = dgw0i[a];
-          pdgwq[a] = dgwqi[a];
-        }
+      for (int b = 0; b < kMaxRef; ++b) {
+        gwj[b] *= gj[b];
+        gw0j[b] *= gj[b];
       }
     }
     const double* blk = t.rc6 + ((size_t)zi * kMaxZ + zj) * kRc6Block;
@@ -376,12 +366,22 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
         va += rc * gwj[b];
         v0a += rc * gw0j[b];
       }
-      c6 += pgw[a] * va;
-      c60 += pgw0[a] * v0a;
-      if (GRAD) {
-        dc6cn += pdgw[a] * va;
-        dc6q += pdgwq[a] * va;
-        dc60cn += pdgw0[a] * v0a;
+      if (S) {
+        c6 += gwi[a] * g[a] * va;
+        c60 += gw0i[a] * g[a] * v0a;
+        if (GRAD) {
+          dc6cn += gwi[a] * dg[a] * va;
+          dc6q += dgwqi[a] * g[a] * va;
+          dc60cn += gw0i[a] * dg[a] * v0a;
+        }
+      } else {
+        c6 += gwi[a] * va;
+        c60 += gw0i[a] * v0a;
+        if (GRAD) {
+          dc6cn += dgwi[a] * va;
+          dc6q += dgwqi[a] * va;
+          dc60cn += dgw0i[a] * v0a;
+        }
       }
     }
     // rational damping, radii = 3 r4r2_i r4r2_j (twobody.py:147-157)
