@@ -135,7 +135,7 @@ def _check_inputs(numbers, positions, charge):
 
 
 def forward(numbers, positions, charge, kp: KernelParams, q=None, r4r2=None,
-            ws=None, q_out=None):
+            ws=None, q_out=None, energy=None):
     """Atom energies (B, N); returns (energy, workspace)."""
     _check_inputs(numbers, positions, charge)
     B, N = numbers.shape
@@ -143,7 +143,8 @@ def forward(numbers, positions, charge, kp: KernelParams, q=None, r4r2=None,
     ctx = context(dev, kp.ga, kp.gc)
     if ws is None:
         ws = new_workspace(B, N, dev)
-    energy = torch.empty((B, N), dtype=torch.float64, device=dev)
+    if energy is None:
+        energy = torch.empty((B, N), dtype=torch.float64, device=dev)
     p = kp.c()
     _lib.check(
         _lib.load().d4_forward(

@@ -5,13 +5,17 @@ Requests (one molecule each: atomic numbers, positions in bohr, total charge,
 energy-only or energy + forces) arrive on the host at any time. The loop
 
 - keeps the element tables resident on the device (one context per device);
-- groups waiting requests by kind and padded size class (multiples of 16
-  atoms, the three-body tile edge) so padding stays small;
-- admits a batch whenever an in-flight slot is free and the batch's device
-  footprint (workspace + inputs + outputs, ``d4_workspace_bytes``) fits in the
-  memory budget next to the batches already on the device;
-- runs every batch on its own HIP stream, so a batch admitted while others
-  are still running overlaps with them on the device;
+- groups waiting requests by kind and padded size bucket (multiples of 32
+  atoms up to 192, the largest size the single-work-group EEQ solver takes,
+  then 256, then multiples of 128), so a batch holds many requests while
+  padding stays moderate (padding rows exit at once in every kernel);
+- runs every batch in one of ``max_inflight`` slots, each with its own HIP
+  stream and its own device arena and pinned host staging buffers, kept
+  between batches (no allocation in steady state); a slot's share of the
+  memory budget bounds the batch size (``d4_workspace_bytes`` + inputs +
+  outputs);
+- packs a batch with vectorised copies into the pinned buffer and moves it
+  with one host-to-device copy, and the results with one copy back;
 - reports each result when its batch's completion event has fired.
 
 The kernels give every molecule a result that does not depend on its batch
@@ -22,8 +26,9 @@ in the mix.
 from __future__ import annotations
 
 import collections
+import gc
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -34,12 +39,20 @@ from .disp import kernel_params
 
 __all__ = ["Request", "Result", "D4Server", "size_class"]
 
-_CLASS_STEP = 16
+_ALIGN = 256
 
 
 def size_class(n: int) -> int:
-    """Padded length of a molecule of n atoms."""
-    return max(_CLASS_STEP, (n + _CLASS_STEP - 1) // _CLASS_STEP * _CLASS_STEP)
+    """Padded length of the batches a molecule of n atoms joins."""
+    if n <= 192:
+        return max(32, (n + 31) // 32 * 32)
+    if n <= 256:
+        return 256
+    return (n + 127) // 128 * 128
+
+
+def _up(x: int) -> int:
+    return (x + _ALIGN - 1) // _ALIGN * _ALIGN
 
 
 @dataclass
@@ -66,51 +79,86 @@ class Result:
         return self.done - self.arrival
 
 
-@dataclass
-class _Batch:
-    bid: int
-    reqs: list
-    stream: torch.cuda.Stream
-    event: torch.cuda.Event
-    energy: torch.Tensor
-    grad: torch.Tensor | None
-    nbytes: int
-    keep: list = field(default_factory=list)
+class _Slot:
+    """One in-flight batch position: a stream, a device arena (inputs,
+    outputs, workspace) and a pinned host buffer (inputs, then outputs),
+    grown on demand and reused."""
+
+    def __init__(self, device: torch.device):
+        self.device = device
+        self.stream = torch.cuda.Stream(device)
+        self.dev = torch.empty(0, dtype=torch.uint8, device=device)
+        self.host = torch.empty(0, dtype=torch.uint8).pin_memory()
+        self.event = None
+        self.reqs = None
+        self.bid = 0
+        self.shape = None
+
+    def reserve(self, dev_bytes: int, host_bytes: int) -> None:
+        if self.dev.numel() < dev_bytes:
+            self.dev = torch.empty(dev_bytes, dtype=torch.uint8, device=self.device)
+        if self.host.numel() < host_bytes:
+            self.host = torch.empty(host_bytes, dtype=torch.uint8).pin_memory()
+
+
+def _layout(nbatch: int, natoms: int, forces: bool) -> dict:
+    """Byte offsets of one batch in a slot: inputs (numbers, positions,
+    charge) and outputs (energy, gradient) in both buffers, the workspace on
+    the device only."""
+    bn = nbatch * natoms
+    off = {}
+    o = 0
+    for name, size in (("numbers", 8 * bn), ("positions", 24 * bn), ("charge", 8 * nbatch)):
+        off[name] = o
+        o = _up(o + size)
+    off["in_end"] = o
+    for name, size in (("energy", 8 * bn), ("grad", 24 * bn if forces else 0)):
+        off[name] = o
+        o = _up(o + size)
+    off["out_end"] = o
+    off["ws"] = o
+    off["ws_bytes"] = runtime.workspace_bytes(nbatch, natoms)
+    off["dev_end"] = o + _up(off["ws_bytes"])
+    return off
+
+
+def _view(buf: torch.Tensor, off: int, dtype, shape) -> torch.Tensor:
+    count = int(np.prod(shape))
+    item = torch.empty(0, dtype=dtype).element_size()
+    return buf[off: off + count * item].view(dtype).view(shape)
 
 
 class D4Server:
     """Continuous batching of D4 requests on one device."""
 
     def __init__(self, param: dict, cutoff: Cutoff | None = None, device="cuda:0",
-                 memory_budget: int = 8 << 30, max_inflight: int = 4,
+                 memory_budget: int = 16 << 30, max_inflight: int = 4,
                  max_batch: int = 1024):
         self.device = torch.device(device)
         self.kp = kernel_params(param, cutoff, None)
         runtime.context(self.device, self.kp.ga, self.kp.gc)  # tables resident now
         self.budget = int(memory_budget)
         self.max_inflight = int(max_inflight)
+        self.slot_budget = self.budget // self.max_inflight
         self.max_batch = int(max_batch)
         self.queues: dict[tuple, collections.deque] = collections.defaultdict(collections.deque)
-        self.inflight: list[_Batch] = []
-        self.in_use = 0
+        self.inflight: list[_Slot] = []
         self.results: list[Result] = []
         self._bid = 0
-        self._streams = [torch.cuda.Stream(self.device) for _ in range(self.max_inflight)]
-        self._free_streams = list(self._streams)
+        self._free = [_Slot(self.device) for _ in range(self.max_inflight)]
 
     # ------------------------------------------------------------ admission
 
     @staticmethod
     def footprint(nbatch: int, natoms: int, forces: bool) -> int:
-        """Device bytes of a batch: workspace + inputs + outputs."""
-        io = natoms * (8 + 24 + 8 + (24 if forces else 0)) + 8
-        return runtime.workspace_bytes(nbatch, natoms) + nbatch * io
+        """Device bytes of a batch: inputs + outputs + workspace."""
+        return _layout(nbatch, natoms, forces)["dev_end"]
 
-    def _admissible(self, npad: int, forces: bool, free: int, limit: int) -> int:
+    def _admissible(self, npad: int, forces: bool, limit: int) -> int:
         lo, hi = 0, limit
         while lo < hi:
             mid = (lo + hi + 1) // 2
-            if self.footprint(mid, npad, forces) <= free:
+            if self.footprint(mid, npad, forces) <= self.slot_budget:
                 lo = mid
             else:
                 hi = mid - 1
@@ -135,39 +183,51 @@ class D4Server:
     def _launch(self, key) -> bool:
         forces, npad = key
         q = self.queues[key]
-        free = self.budget - self.in_use
-        count = self._admissible(npad, forces, free, min(len(q), self.max_batch))
+        count = self._admissible(npad, forces, min(len(q), self.max_batch))
         if count == 0:
-            if not self.inflight and self.footprint(1, npad, forces) > self.budget:
-                raise MemoryError(f"a {npad}-atom request exceeds the memory budget")
-            return False
+            raise MemoryError(f"a {npad}-atom request exceeds the per-slot memory budget "
+                              f"({self.slot_budget} bytes)")
         reqs = [q.popleft() for _ in range(count)]
-        stream = self._free_streams.pop()
-        numbers = np.zeros((count, npad), dtype=np.int64)
-        positions = np.zeros((count, npad, 3), dtype=np.float64)
-        charge = np.zeros((count,), dtype=np.float64)
-        for k, r in enumerate(reqs):
-            numbers[k, : len(r.numbers)] = r.numbers
-            positions[k, : len(r.numbers)] = r.positions
-            charge[k] = r.charge
-        with torch.cuda.stream(stream):
-            nt = torch.from_numpy(numbers).pin_memory().to(self.device, non_blocking=True)
-            pt = torch.from_numpy(positions).pin_memory().to(self.device, non_blocking=True)
-            ct = torch.from_numpy(charge).pin_memory().to(self.device, non_blocking=True)
+        slot = self._free.pop()
+        lay = _layout(count, npad, forces)
+        slot.reserve(lay["dev_end"], lay["out_end"])
+        # pack into the pinned buffer: zero padding, then scatter every
+        # request's atoms to (row, column) in one vectorised copy
+        lens = np.fromiter((len(r.numbers) for r in reqs), dtype=np.int64, count=count)
+        rows = np.repeat(np.arange(count), lens)
+        cols = np.arange(int(lens.sum())) - np.repeat(np.cumsum(lens) - lens, lens)
+        h = slot.host.numpy()
+        hn = h[lay["numbers"]: lay["numbers"] + 8 * count * npad].view(np.int64)
+        hn = hn.reshape(count, npad)
+        hp = h[lay["positions"]: lay["positions"] + 24 * count * npad].view(np.float64)
+        hp = hp.reshape(count, npad, 3)
+        hc = h[lay["charge"]: lay["charge"] + 8 * count].view(np.float64)
+        hn.fill(0)
+        hp.fill(0.0)
+        hn[rows, cols] = np.concatenate([r.numbers for r in reqs])
+        hp[rows, cols] = np.concatenate([r.positions for r in reqs])
+        hc[:] = [r.charge for r in reqs]
+        with torch.cuda.stream(slot.stream):
+            slot.dev[: lay["in_end"]].copy_(slot.host[: lay["in_end"]], non_blocking=True)
+            nt = _view(slot.dev, lay["numbers"], torch.int64, (count, npad))
+            pt = _view(slot.dev, lay["positions"], torch.float64, (count, npad, 3))
+            ct = _view(slot.dev, lay["charge"], torch.float64, (count,))
+            ws = slot.dev[lay["ws"]: lay["ws"] + lay["ws_bytes"]]
+            energy = _view(slot.dev, lay["energy"], torch.float64, (count, npad))
             if forces:
-                energy, grad, ws = runtime.energy_gradient(nt, pt, ct, self.kp)
+                grad = _view(slot.dev, lay["grad"], torch.float64, (count, npad, 3))
+                runtime.energy_gradient(nt, pt, ct, self.kp, ws=ws, energy=energy, grad=grad)
             else:
-                energy, ws = runtime.forward(nt, pt, ct, self.kp)
-                grad = None
-            e_host = energy.to("cpu", non_blocking=True)
-            g_host = grad.to("cpu", non_blocking=True) if forces else None
-            event = torch.cuda.Event()
-            event.record(stream)
-        nbytes = self.footprint(count, npad, forces)
-        self.in_use += nbytes
+                runtime.forward(nt, pt, ct, self.kp, ws=ws, energy=energy)
+            lo, hi = lay["energy"], lay["out_end"]
+            slot.host[lo:hi].copy_(slot.dev[lo:hi], non_blocking=True)
+            slot.event = torch.cuda.Event()
+            slot.event.record(slot.stream)
         self._bid += 1
-        self.inflight.append(_Batch(self._bid, reqs, stream, event, e_host, g_host, nbytes,
-                                    keep=[nt, pt, ct, energy, grad, ws]))
+        slot.bid = self._bid
+        slot.reqs = reqs
+        slot.shape = (count, npad, forces, lay)
+        self.inflight.append(slot)
         return True
 
     # ------------------------------------------------------------------ loop
@@ -176,52 +236,65 @@ class D4Server:
         """Collect finished batches; returns the number of results added."""
         added = 0
         still = []
-        for b in self.inflight:
-            if b.event.query():
-                now = time.perf_counter()
-                e = b.energy.numpy()
-                g = b.grad.numpy() if b.grad is not None else None
-                for k, r in enumerate(b.reqs):
-                    n = len(r.numbers)
-                    self.results.append(Result(r.rid, e[k, :n].copy(),
-                                               None if g is None else g[k, :n].copy(),
-                                               r.arrival, now, b.bid))
-                    added += 1
-                self.in_use -= b.nbytes
-                self._free_streams.append(b.stream)
-            else:
-                still.append(b)
+        for slot in self.inflight:
+            if not slot.event.query():
+                still.append(slot)
+                continue
+            now = time.perf_counter()
+            count, npad, forces, lay = slot.shape
+            h = slot.host.numpy()
+            e = h[lay["energy"]: lay["energy"] + 8 * count * npad].view(np.float64)
+            e = e.reshape(count, npad)
+            g = None
+            if forces:
+                g = h[lay["grad"]: lay["grad"] + 24 * count * npad].view(np.float64)
+                g = g.reshape(count, npad, 3)
+            for k, r in enumerate(slot.reqs):
+                n = len(r.numbers)
+                self.results.append(Result(r.rid, e[k, :n].copy(),
+                                           None if g is None else g[k, :n].copy(),
+                                           r.arrival, now, slot.bid))
+                added += 1
+            slot.reqs = None
+            self._free.append(slot)
         self.inflight = still
         return added
 
     def step(self) -> None:
-        """Poll completions, then admit as many batches as slots and memory
-        allow."""
+        """Poll completions, then admit as many batches as free slots allow."""
         self.poll()
-        while self._free_streams and self.pending():
-            key = self._pick_queue()
-            if not self._launch(key):
-                break
+        while self._free and self.pending():
+            self._launch(self._pick_queue())
 
     def drain(self) -> None:
         while self.pending() or self.inflight:
             self.step()
 
     def run_stream(self, requests: list[Request], arrivals: np.ndarray) -> list[Result]:
-        """Replay requests at the given arrival offsets (seconds from now)
-        and serve them to completion."""
+        """Replay requests at the given arrival offsets (seconds from now),
+        serve them to completion and return their results. The cyclic
+        garbage collector is paused meanwhile (a full collection over the
+        request objects takes tens of milliseconds, longer than most requests
+        take to serve)."""
+        first = len(self.results)
         order = np.argsort(arrivals, kind="stable")
-        t0 = time.perf_counter()
-        i = 0
-        while i < len(order) or self.pending() or self.inflight:
-            now = time.perf_counter() - t0
-            while i < len(order) and arrivals[order[i]] <= now:
-                r = requests[order[i]]
-                r.arrival = t0 + arrivals[order[i]]
-                self.submit(r)
-                i += 1
-            self.step()
-        return self.results
+        was_enabled = gc.isenabled()
+        gc.disable()
+        try:
+            t0 = time.perf_counter()
+            i = 0
+            while i < len(order) or self.pending() or self.inflight:
+                now = time.perf_counter() - t0
+                while i < len(order) and arrivals[order[i]] <= now:
+                    r = requests[order[i]]
+                    r.arrival = t0 + arrivals[order[i]]
+                    self.submit(r)
+                    i += 1
+                self.step()
+        finally:
+            if was_enabled:
+                gc.enable()
+        return self.results[first:]
 
 
 def latency_stats(results: list[Result]) -> dict:
