@@ -14,7 +14,7 @@ hand-written FP64 HIP kernels for gfx950 behind a C ABI
 from .__version__ import __version__
 from .cutoff import Cutoff
 from .damping import Damping, Param, RationalDamping, ZeroDamping
-from .disp import dftd4, dftd4_energy_gradient, get_properties
+from .disp import dftd4, dftd4_energy_gradient, dftd4_hessian, get_properties
 from .model import D4Model, D4SModel
 from .ncoord import cn_d4, erf_count
 
@@ -30,6 +30,7 @@ __all__ = [
     "cn_d4",
     "dftd4",
     "dftd4_energy_gradient",
+    "dftd4_hessian",
     "erf_count",
     "get_properties",
 ]

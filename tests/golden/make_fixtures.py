@@ -14,6 +14,10 @@ Sources (relative to the reference checkout):
   ``disp3`` (dftd4 Fortran test suite values).
 - ``test/test_grad/samples_grad.py`` LiH and SiH4 ``grad`` (tblite, TPSS0
   parameters of ``test/test_grad/test_pos.py:55-62``).
+- ``test/test_grad/samples_hessian.py`` LiH and SiH4 ``hessian`` (dftd4 3.6.0,
+  finite differences of the analytic gradient, TPSS0 parameters of
+  ``test/test_grad/test_hessian.py:84-90``; flat, Fortran order of
+  ``(nat, 3, nat, 3)``, ``test_hessian.py:92-95``; reference tolerance 1e-7).
 - ``test/test_model/samples.py`` LiH and SiH4 ``gw`` (Fortran-ordered
   ``(nref, nat)``) and ``c6``; the D4S ``gw_d4s`` (Fortran-ordered
   ``(nref, nat, nat)``) and ``c6_d4s``.
@@ -108,6 +112,7 @@ def build(ref: Path) -> dict:
     init_doc = (ref / "src" / "tad_dftd4" / "__init__.py").read_text()
     d4s = _strip((ref / "test" / "test_d4" / "samples.py").read_text())
     grads = _strip((ref / "test" / "test_grad" / "samples_grad.py").read_text())
+    hess = _strip((ref / "test" / "test_grad" / "samples_hessian.py").read_text())
     models = _strip((ref / "test" / "test_model" / "samples.py").read_text())
 
     out: dict = {"provenance": __doc__.strip().splitlines()[0]}
@@ -148,6 +153,7 @@ def build(ref: Path) -> dict:
         b = _block(d4s, name)
         g = _block(grads, name)
         m = _block(models, name)
+        h = _block(hess, name)
         out[name] = {
             **RESTATED_GEOMETRY[name],
             "q": _entry(b, "q"),
@@ -155,6 +161,7 @@ def build(ref: Path) -> dict:
             "disp2": _entry(b, "disp2"),
             "disp3": _entry(b, "disp3"),
             "grad": _entry(g, "grad"),
+            "hessian": _entry(h, "hessian"),
             "gw": _entry(m, "gw"),
             "c6": _entry(m, "c6"),
             "disp2_d4s": _entry(b, "disp2_d4s"),

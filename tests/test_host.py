@@ -134,3 +134,16 @@ def test_cutoff_and_model_objects():
     assert not d4.disp.kernel_params({"a1": 0.4, "a2": 5.0}, None, m).d4s
     t = d4.tables.device_tables()
     assert t.wfpair.shape == (104, 104) and t.wfpair[6, 1] == t.wfpair[1, 6]
+
+
+def test_five_point_stencil():
+    """The Hessian stencil: (8 (f(h) - f(-h)) - (f(2h) - f(-2h))) / 12h is
+    exact for polynomials up to degree 4 and O(h^4) otherwise."""
+    from tad_dftd4_amd import disp
+
+    h = disp.HESSIAN_STEP
+    x = torch.tensor([s * h for s in disp._STENCIL], dtype=torch.float64)
+    for c in ([1.0, 2.0, 3.0, 4.0, 5.0], [0.5, -1.0, 0.0, 7.0, -2.0]):
+        f = sum(ci * x**i for i, ci in enumerate(c))
+        assert abs(disp._five_point(f).item() - c[1]) < 1e-9
+    assert abs(disp._five_point(torch.sin(1.0 + x)).item() - np.cos(1.0)) < 1e-12

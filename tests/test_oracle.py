@@ -427,3 +427,20 @@ def test_exceptional_weights_reference_gradient_is_nan():
         fd = (dense.dftd4(numbers, pp, mol.charge, par, cutoff=cut).sum()
               - dense.dftd4(numbers, pm, mol.charge, par, cutoff=cut).sum()) / (2 * h)
         assert abs(fd.item() - g_safe[i, x].item()) < 1e-9
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4"])
+def test_hessian_fixture(name):
+    """dftd4 3.6.0 Hessians (test/test_grad/samples_hessian.py, TPSS0,
+    Fortran order, test_grad/test_hessian.py:73-103; reference tolerance
+    1e-7) against the oracle's autograd Hessian, which is symmetric and has
+    zero row sums (translation invariance)."""
+    numbers, positions = _mol(name)
+    n = len(numbers)
+    par = _p(GOLD["param_tpss0"])
+    h = torch.autograd.functional.hessian(
+        lambda p: dense.dftd4(numbers, p, 0.0, par).sum(), positions).reshape(n, 3, n, 3)
+    ref = torch.tensor(GOLD[name]["hessian"], dtype=F64).reshape(3, n, 3, n).permute(3, 2, 1, 0)
+    assert (h - ref).abs().max().item() < 1e-9
+    assert (h - h.permute(2, 3, 0, 1)).abs().max().item() < 1e-18
+    assert h.sum(dim=2).abs().max().item() < 1e-15
