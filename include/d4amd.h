@@ -43,6 +43,7 @@ extern "C" {
 #define D4_FLAG_FORCE_PAIR_TILES 16 /* testing: three-body term by pair tiles at any size */
 #define D4_FLAG_D4S 32     /* D4S model: Gaussian weights with pairwise factors
                               (reference src/tad_dftd4/model/d4s.py:56-285) */
+#define D4_FLAG_ATM16 64   /* testing: one-pass three-body kernel with 16 x 16 pair tiles */
 
 /* Element and reference-system tables, host memory, read once by
  * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference

@@ -52,6 +52,7 @@ struct Params {
   int force_pairtiles;
   int pgrad;  // accumulate parameter gradients (backward only)
   int d4s;    // D4S model: Gaussian weights with pairwise factors
+  int atm16;  // one-pass three-body kernel with 16 x 16 pair tiles (testing)
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).
@@ -109,12 +110,28 @@ __host__ __device__ inline int64_t n_groups(int64_t N) { return (N + kGroup - 1)
 constexpr int kAtm1MaxN = 256;
 constexpr int kAtm1Acc = 5;  // energy share, gradient (3), dL/dCN
 __host__ __device__ inline int atm1_splits(int n);
+// The 8 x 8 variant (one wave per work-group, kAtm8-atom blocks; the
+// gradient passes) splits a molecule over atm8_splits(n) waves, also a
+// function of n alone: about one 32-atom molecule of work each, at most 64
+// (each split writes a partial sum per atom).
+constexpr int kAtm8 = 8;
+constexpr int kAtm8MaxSplits = 64;
+__host__ __device__ inline int atm8_splits(int n) {
+  const int nb = (n + kAtm8 - 1) / kAtm8;
+  const int tiles = nb * (nb + 1) / 2;
+  const long long work = (long long)n * n * n;
+  int s = (int)((work + 32767) / 32768);
+  s = s > kAtm8MaxSplits ? kAtm8MaxSplits : s;
+  return tiles < 1 ? 1 : (s < 1 ? 1 : (s > tiles ? tiles : s));
+}
 // Three-body parameter-gradient partials per molecule: one per split of the
 // one-pass kernel or one per pair tile of the pair-tile kernel (the larger).
 __host__ __device__ inline int64_t pgrad_partials(int64_t N) {
   const int64_t nb = n_groups(N);
   const int64_t tiles = nb * (nb + 1) / 2;
-  const int64_t s = N <= kAtm1MaxN ? atm1_splits((int)N) : 0;
+  int64_t s = N <= kAtm1MaxN ? atm1_splits((int)N) : 0;
+  const int64_t s8 = N <= kAtm1MaxN ? atm8_splits((int)N) : 0;
+  s = s > s8 ? s : s8;
   return tiles > s ? tiles : s;
 }
 __host__ __device__ inline int atm1_splits(int n) {
@@ -124,6 +141,24 @@ __host__ __device__ inline int atm1_splits(int n) {
   const int s = (int)((work + 262143) / 262144);  // about one 64-atom molecule each
   return tiles < 1 ? 1 : (s < 1 ? 1 : (s > tiles ? tiles : s));
 }
+__host__ __device__ inline int one_pass_split_max(int n) {
+  const int a = atm1_splits(n), b = atm8_splits(n);
+  return a > b ? a : b;
+}
+
+// Small molecules take the one-pass three-body kernel.
+inline bool one_pass(const Params& p, const Work& w) {
+  return w.N <= kAtm1MaxN && !p.force_pairtiles;
+}
+
+// Splits of a molecule of n atoms in the one-pass kernel in use.
+__host__ __device__ inline int one_pass_splits(int n, bool tile8) {
+  return tile8 ? atm8_splits(n) : atm1_splits(n);
+}
+
+// The gradient passes use the 8 x 8 one-pass kernel, energies alone the
+// 16 x 16 one (fewer, larger work-groups suit the lighter energy slots).
+inline bool one_pass_tile8(const Params& p, bool grad) { return grad && !p.atm16; }
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -179,7 +214,8 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
   v.sG = (double*)take(24 * bn);
   v.gsph = (double*)take(32 * size_t(B) * size_t(n_groups(N)));
   v.apart = (double*)take(N <= kAtm1MaxN
-                              ? 8 * size_t(B) * size_t(atm1_splits((int)N)) * kAtm1Acc * size_t(N)
+                              ? 8 * size_t(B) * size_t(one_pass_split_max((int)N)) * kAtm1Acc *
+                                    size_t(N)
                               : 0);
   v.rt4 = (double*)take(8 * bn);
   v.pg2 = (double*)take(8 * bn * 5);

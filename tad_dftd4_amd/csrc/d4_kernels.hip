@@ -260,14 +260,189 @@ __device__ __forceinline__ void pair_gauss(const DevTables& t, int z, int nr, do
 // --------------------------------------------------------------------------
 // pair pass: C6(q) and C6(0) contraction with the reference-C6 table, the
 // rational-damped two-body term, the ATM pair factors; with GRAD also the
-// two-body gradient pieces (one wave per atom row). With S (D4S model) the
-// weights of both atoms are formed per pair from the charge scaling that
-// k_weights left and the pair's Gaussian weights.
+// two-body gradient pieces. With S (D4S model) the weights of both atoms are
+// formed per pair from the charge scaling that k_weights left and the pair's
+// Gaussian weights.
 // --------------------------------------------------------------------------
 
+// Atom i of a pair (the atom whose sums are accumulated).
+struct PairAtom {
+  int z, nr;
+  double x, y, zc, r4, cn, g;
+  // D4: weights and their derivatives; D4S: zeta(q), zeta(0), dzeta/dq
+  double gw[kMaxRef], gw0[kMaxRef], dgw[kMaxRef], dgw0[kMaxRef], dgwq[kMaxRef];
+};
+
+struct PairSums {
+  double e = 0.0, cn = 0.0, q = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+  // parameter gradients of sum_i gE_i E2_i: d/ds6, d/ds8, d/ds10, d/da1, d/da2
+  double s6 = 0.0, s8 = 0.0, s10 = 0.0, a1 = 0.0, a2 = 0.0;
+};
+
+template <bool GRAD, bool S>
+__device__ __forceinline__ void load_pair_atom(const DevTables& t, const Work& w,
+                                               const Inputs& in, int64_t row, int zi,
+                                               const double* R, int i, PairAtom& A) {
+  A.z = zi;
+  A.nr = t.nref[zi];
+  A.x = R[3 * i];
+  A.y = R[3 * i + 1];
+  A.zc = R[3 * i + 2];
+  A.r4 = in.r4r2_in ? in.r4r2_in[row] : t.r4r2[zi];
+  A.cn = S ? w.cn[row] : 0.0;
+  A.g = GRAD ? (in.gE ? in.gE[row] : 1.0) : 0.0;
+#pragma unroll
+  for (int a = 0; a < kMaxRef; ++a) {
+    A.gw[a] = w.gw[row * kRefStride + a];
+    A.gw0[a] = w.gw0[row * kRefStride + a];
+    if (GRAD) {
+      A.dgw[a] = S ? 0.0 : w.dgw[row * kRefStride + a];
+      A.dgw0[a] = S ? 0.0 : w.dgw0[row * kRefStride + a];
+      A.dgwq[a] = w.dgwq[row * kRefStride + a];
+    }
+  }
+}
+
+// One pair (i, j): accumulates atom i's two-body energy, dL/dCN_i, dL/dq_i,
+// forces and parameter partials, and returns the ATM pair factors
+// tf = (R0/r)^(alp/3), pf = sqrt|C6(0)|/r^3 and pk = d log sqrt|C6(0)| / d CN_i.
+template <bool GRAD, bool PG, bool S>
+__device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, const PairAtom& A,
+                                          int zj, double xj, double yj, double zzj, double r4j,
+                                          double cnj, double gj, const double* gwjp,
+                                          const double* gw0jp, PairSums& acc, double& tf,
+                                          double& pf, double& pk) {
+  const double dx = A.x - xj, dy = A.y - yj, dz = A.zc - zzj;
+  const double r2 = dx * dx + dy * dy + dz * dz;
+  const double r = sqrt(r2);
+  double gwj[kMaxRef], gw0j[kMaxRef];
+#pragma unroll
+  for (int b = 0; b < kMaxRef; ++b) {
+    gwj[b] = gwjp[b];
+    gw0j[b] = gw0jp[b];
+  }
+  // D4S: Gaussian weights of both atoms for this pair; atom n's weights for
+  // partner m use wf(Z_n, Z_m) (model/d4s.py:196). The weights of atom i
+  // are zeta_i[a] g[a] (A.gw, A.gw0, A.dgwq hold zeta).
+  double g[kMaxRef], dg[kMaxRef];
+  if (S) {
+    pair_gauss<GRAD>(t, A.z, A.nr, A.cn, t.wfpair[A.z * kMaxZ + zj], g, dg);
+    double gjw[kMaxRef];
+    pair_gauss<false>(t, zj, t.nref[zj], cnj, t.wfpair[zj * kMaxZ + A.z], gjw, nullptr);
+#pragma unroll
+    for (int b = 0; b < kMaxRef; ++b) {
+      gwj[b] *= gjw[b];
+      gw0j[b] *= gjw[b];
+    }
+  }
+  const double* blk = t.rc6 + ((size_t)A.z * kMaxZ + zj) * kRc6Block;
+  double c6 = 0.0, c60 = 0.0, dc6cn = 0.0, dc6q = 0.0, dc60cn = 0.0;
+#pragma unroll
+  for (int a = 0; a < kMaxRef; ++a) {
+    if (a >= A.nr) continue;
+    double va = 0.0, v0a = 0.0;
+#pragma unroll
+    for (int b = 0; b < kMaxRef; ++b) {
+      const double rc = blk[a * kMaxRef + b];
+      va += rc * gwj[b];
+      v0a += rc * gw0j[b];
+    }
+    if (S) {
+      c6 += A.gw[a] * g[a] * va;
+      c60 += A.gw0[a] * g[a] * v0a;
+      if (GRAD) {
+        dc6cn += A.gw[a] * dg[a] * va;
+        dc6q += A.dgwq[a] * g[a] * va;
+        dc60cn += A.gw0[a] * dg[a] * v0a;
+      }
+    } else {
+      c6 += A.gw[a] * va;
+      c60 += A.gw0[a] * v0a;
+      if (GRAD) {
+        dc6cn += A.dgw[a] * va;
+        dc6q += A.dgwq[a] * va;
+        dc60cn += A.dgw0[a] * v0a;
+      }
+    }
+  }
+  // rational damping, radii = 3 r4r2_i r4r2_j (twobody.py:147-157)
+  const double qq = 3.0 * A.r4 * r4j;
+  const double r0 = p.a1 * sqrt(qq) + p.a2;
+  const double r6 = r2 * r2 * r2, r8 = r6 * r2;
+  const double r06 = r0 * r0 * r0 * r0 * r0 * r0, r08 = r06 * r0 * r0;
+  const double t6 = 1.0 / (r6 + r06);
+  const double t8 = 1.0 / (r8 + r08);
+  double de_dc6 = p.s6 * t6 + p.s8 * qq * t8;
+  double t10 = 0.0, r10 = 0.0;
+  if (p.s10on) {
+    r10 = r8 * r2;
+    t10 = 1.0 / (r10 + r08 * r0 * r0);
+    de_dc6 += p.s10 * qq * qq * (49.0 / 40.0) * t10;
+  }
+  de_dc6 *= -0.5;
+  const bool in2 = r <= p.cut2;
+  if (in2) acc.e += de_dc6 * c6;
+  if (PG && in2) {
+    const double ce = -0.5 * c6 * A.g;
+    acc.s6 = fma(ce, t6, acc.s6);
+    acc.s8 = fma(ce * qq, t8, acc.s8);
+    // d/dr0 of s6 t6 + s8 qq t8 (+ s10 ...): d t_n / d r0 = -n r0^(n-1) t_n^2
+    double dr0 = p.s6 * (-6.0 * r06 / r0) * t6 * t6 + p.s8 * qq * (-8.0 * r08 / r0) * t8 * t8;
+    if (p.s10on) {
+      acc.s10 = fma(ce * qq * qq * (49.0 / 40.0), t10, acc.s10);
+      dr0 += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r08 * r0) * t10 * t10;
+    }
+    acc.a1 = fma(ce * dr0, sqrt(qq), acc.a1);
+    acc.a2 = fma(ce, dr0, acc.a2);
+  }
+  // ATM pair factors (threebody.py:106-145, 241-256, 311-321)
+  tf = pow(r0 / r, p.alp / 3.0);
+  pf = sqrt(fabs(c60)) / (r2 * r);
+  pk = 0.0;
+  if (GRAD) {
+    pk = c60 != 0.0 ? 0.5 * dc60cn / c60 : 0.0;
+    if (in2) {
+      const double wij = A.g + gj;
+      acc.cn += wij * de_dc6 * dc6cn;
+      acc.q += wij * de_dc6 * dc6q;
+      // d/dr of 1/(r^n + r0^n) = -n r^(n-1) t^2
+      double dedr = p.s6 * (-6.0 * r6 / r) * t6 * t6 + p.s8 * qq * (-8.0 * r8 / r) * t8 * t8;
+      if (p.s10on) dedr += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r10 / r) * t10 * t10;
+      dedr *= -0.5 * c6;
+      const double coef = wij * dedr / r;
+      acc.fx += coef * dx;
+      acc.fy += coef * dy;
+      acc.fz += coef * dz;
+    }
+  }
+}
+
+template <bool GRAD, bool PG>
+__device__ __forceinline__ void store_pair_atom(const Work& w, int64_t row, double r4,
+                                                const PairSums& acc, double* grad_out) {
+  w.e2[row] = acc.e;
+  if (GRAD) {
+    w.dldcn[row] = acc.cn;
+    w.dldq[row] = acc.q;
+    grad_out[3 * row] = acc.fx;
+    grad_out[3 * row + 1] = acc.fy;
+    grad_out[3 * row + 2] = acc.fz;
+  }
+  if (PG) {
+    w.rt4[row] = sqrt(r4);
+    w.pg2[row * 5 + 0] = acc.s6;
+    w.pg2[row * 5 + 1] = acc.s8;
+    w.pg2[row * 5 + 2] = acc.s10;
+    w.pg2[row * 5 + 3] = acc.a1;
+    w.pg2[row * 5 + 4] = acc.a2;
+  }
+}
+
+// RL lanes per atom row i walk the partners j; pt[i][j], pp[i][j] and
+// pk[i][j] are written along the row.
 template <bool GRAD, bool PG, bool S, int RL>
-__global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
-                                               Work w, double* grad_out) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pairs(
+    DevTables t, Params p, Inputs in, Work w, double* grad_out) {
   const int lane = threadIdx.x & (RL - 1);
   const int64_t row = (int64_t)blockIdx.x * (256 / RL) + threadIdx.x / RL;
   if (row >= w.B * w.N) return;
@@ -287,44 +462,18 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
       ppr[j] = 0.0;
       if (GRAD) pkr[j] = 0.0;
     }
-    if (lane == 0) {
-      w.e2[row] = 0.0;
-      if (GRAD) {
-        w.dldcn[row] = 0.0;
-        w.dldq[row] = 0.0;
-        grad_out[3 * row] = 0.0;
-        grad_out[3 * row + 1] = 0.0;
-        grad_out[3 * row + 2] = 0.0;
-      }
-      if (PG) {
-        w.rt4[row] = 0.0;
-        for (int k = 0; k < 5; ++k) w.pg2[row * 5 + k] = 0.0;
-      }
-    }
+    if (lane == 0) store_pair_atom<GRAD, PG>(w, row, 0.0, PairSums{}, grad_out);
     return;
   }
-  const double xi = R[3 * i], yi = R[3 * i + 1], zzi = R[3 * i + 2];
-  const double r4i = in.r4r2_in ? in.r4r2_in[row] : t.r4r2[zi];
-  const int nri = t.nref[zi];
-  double gwi[kMaxRef], gw0i[kMaxRef], dgwi[kMaxRef], dgw0i[kMaxRef], dgwqi[kMaxRef];
-#pragma unroll
-  for (int a = 0; a < kMaxRef; ++a) {
-    gwi[a] = w.gw[row * kRefStride + a];
-    gw0i[a] = w.gw0[row * kRefStride + a];
-    if (GRAD) {
-      dgwi[a] = S ? 0.0 : w.dgw[row * kRefStride + a];
-      dgw0i[a] = S ? 0.0 : w.dgw0[row * kRefStride + a];
-      dgwqi[a] = w.dgwq[row * kRefStride + a];
-    }
-  }
-  const double cni = S ? w.cn[row] : 0.0;
-  const double gi = GRAD ? (in.gE ? in.gE[row] : 1.0) : 0.0;
-  const double alp3 = p.alp / 3.0;
-  double acc_e = 0.0, acc_cn = 0.0, acc_q = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
-  // parameter gradients of sum_i gE_i E2_i: d/ds6, d/ds8, d/ds10, d/da1, d/da2
-  double g_s6 = 0.0, g_s8 = 0.0, g_s10 = 0.0, g_a1 = 0.0, g_a2 = 0.0;
+  PairAtom A;
+  load_pair_atom<GRAD, S>(t, w, in, row, zi, R, i, A);
+  PairSums acc;
+  // the partner's Z is loaded one iteration ahead: the table reads of a pair
+  // (reference C6 block, r4r2) depend on it
+  int znext = lane < n ? (int)Z[lane] : 0;
   for (int j = lane; j < n; j += RL) {
-    const int zj = (int)Z[j];
+    const int zj = znext;
+    znext = j + RL < n ? (int)Z[j + RL] : 0;
     if (j == i || zj == 0) {
       ptr[j] = 0.0;
       ppr[j] = 0.0;
@@ -332,145 +481,32 @@ __global__ __launch_bounds__(256) void k_pairs(DevTables t, Params p, Inputs in,
       continue;
     }
     const int64_t rj = m * N + j;
-    const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double r = sqrt(r2);
     const double r4j = in.r4r2_in ? in.r4r2_in[rj] : t.r4r2[zj];
-    double gwj[kMaxRef], gw0j[kMaxRef];
-#pragma unroll
-    for (int b = 0; b < kMaxRef; ++b) {
-      gwj[b] = w.gw[rj * kRefStride + b];
-      gw0j[b] = w.gw0[rj * kRefStride + b];
-    }
-    // D4S: Gaussian weights of both atoms for this pair; atom n's weights
-    // for partner m use wf(Z_n, Z_m) (model/d4s.py:196). The weights of
-    // atom i are zeta_i[a] g[a] (row arrays gwi, gw0i, dgwqi hold zeta).
-    double g[kMaxRef], dg[kMaxRef];
-    if (S) {
-      pair_gauss<GRAD>(t, zi, nri, cni, t.wfpair[zi * kMaxZ + zj], g, dg);
-      double gj[kMaxRef];
-      pair_gauss<false>(t, zj, t.nref[zj], w.cn[rj], t.wfpair[zj * kMaxZ + zi], gj, nullptr);
-#pragma unroll
-      for (int b = 0; b < kMaxRef; ++b) {
-        gwj[b] *= gj[b];
-        gw0j[b] *= gj[b];
-      }
-    }
-    const double* blk = t.rc6 + ((size_t)zi * kMaxZ + zj) * kRc6Block;
-    double c6 = 0.0, c60 = 0.0, dc6cn = 0.0, dc6q = 0.0, dc60cn = 0.0;
-    for (int a = 0; a < nri; ++a) {
-      double va = 0.0, v0a = 0.0;
-#pragma unroll
-      for (int b = 0; b < kMaxRef; ++b) {
-        const double rc = blk[a * kMaxRef + b];
-        va += rc * gwj[b];
-        v0a += rc * gw0j[b];
-      }
-      if (S) {
-        c6 += gwi[a] * g[a] * va;
-        c60 += gw0i[a] * g[a] * v0a;
-        if (GRAD) {
-          dc6cn += gwi[a] * dg[a] * va;
-          dc6q += dgwqi[a] * g[a] * va;
-          dc60cn += gw0i[a] * dg[a] * v0a;
-        }
-      } else {
-        c6 += gwi[a] * va;
-        c60 += gw0i[a] * v0a;
-        if (GRAD) {
-          dc6cn += dgwi[a] * va;
-          dc6q += dgwqi[a] * va;
-          dc60cn += dgw0i[a] * v0a;
-        }
-      }
-    }
-    // rational damping, radii = 3 r4r2_i r4r2_j (twobody.py:147-157)
-    const double qq = 3.0 * r4i * r4j;
-    const double r0 = p.a1 * sqrt(qq) + p.a2;
-    const double r6 = r2 * r2 * r2, r8 = r6 * r2;
-    const double r06 = r0 * r0 * r0 * r0 * r0 * r0, r08 = r06 * r0 * r0;
-    const double t6 = 1.0 / (r6 + r06);
-    const double t8 = 1.0 / (r8 + r08);
-    double de_dc6 = p.s6 * t6 + p.s8 * qq * t8;
-    double t10 = 0.0, r10 = 0.0;
-    if (p.s10on) {
-      r10 = r8 * r2;
-      t10 = 1.0 / (r10 + r08 * r0 * r0);
-      de_dc6 += p.s10 * qq * qq * (49.0 / 40.0) * t10;
-    }
-    de_dc6 *= -0.5;
-    const bool in2 = r <= p.cut2;
-    if (in2) acc_e += de_dc6 * c6;
-    if (PG && in2) {
-      const double ce = -0.5 * c6 * gi;
-      g_s6 = fma(ce, t6, g_s6);
-      g_s8 = fma(ce * qq, t8, g_s8);
-      // d/dr0 of s6 t6 + s8 qq t8 (+ s10 ...): d t_n / d r0 = -n r0^(n-1) t_n^2
-      double dr0 = p.s6 * (-6.0 * r06 / r0) * t6 * t6 + p.s8 * qq * (-8.0 * r08 / r0) * t8 * t8;
-      if (p.s10on) {
-        g_s10 = fma(ce * qq * qq * (49.0 / 40.0), t10, g_s10);
-        dr0 += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r08 * r0) * t10 * t10;
-      }
-      g_a1 = fma(ce * dr0, sqrt(qq), g_a1);
-      g_a2 = fma(ce, dr0, g_a2);
-    }
-    // ATM pair factors (threebody.py:106-145, 241-256, 311-321)
-    const double tf = pow(r0 / r, alp3);
-    const double pf = sqrt(fabs(c60)) / (r2 * r);
+    const double cnj = S ? w.cn[rj] : 0.0;
+    const double gj = GRAD ? (in.gE ? in.gE[rj] : 1.0) : 0.0;
+    double tf, pf, pk;
+    pair_eval<GRAD, PG, S>(t, p, A, zj, R[3 * j], R[3 * j + 1], R[3 * j + 2], r4j, cnj, gj,
+                           w.gw + rj * kRefStride, w.gw0 + rj * kRefStride, acc, tf, pf, pk);
     ptr[j] = tf;
     ppr[j] = pf;
-    if (GRAD) {
-      pkr[j] = c60 != 0.0 ? 0.5 * dc60cn / c60 : 0.0;
-      if (in2) {
-        const double gj = in.gE ? in.gE[rj] : 1.0;
-        const double wij = gi + gj;
-        acc_cn += wij * de_dc6 * dc6cn;
-        acc_q += wij * de_dc6 * dc6q;
-        // d/dr of 1/(r^n + r0^n) = -n r^(n-1) t^2
-        double dedr = p.s6 * (-6.0 * r6 / r) * t6 * t6 +
-                      p.s8 * qq * (-8.0 * r8 / r) * t8 * t8;
-        if (p.s10on) dedr += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r10 / r) * t10 * t10;
-        dedr *= -0.5 * c6;
-        const double coef = wij * dedr / r;
-        fx += coef * dx;
-        fy += coef * dy;
-        fz += coef * dz;
-      }
-    }
+    if (GRAD) pkr[j] = pk;
   }
-  acc_e = row_sum<RL>(acc_e);
+  acc.e = row_sum<RL>(acc.e);
   if (GRAD) {
-    acc_cn = row_sum<RL>(acc_cn);
-    acc_q = row_sum<RL>(acc_q);
-    fx = row_sum<RL>(fx);
-    fy = row_sum<RL>(fy);
-    fz = row_sum<RL>(fz);
+    acc.cn = row_sum<RL>(acc.cn);
+    acc.q = row_sum<RL>(acc.q);
+    acc.fx = row_sum<RL>(acc.fx);
+    acc.fy = row_sum<RL>(acc.fy);
+    acc.fz = row_sum<RL>(acc.fz);
   }
   if (PG) {
-    g_s6 = row_sum<RL>(g_s6);
-    g_s8 = row_sum<RL>(g_s8);
-    g_s10 = row_sum<RL>(g_s10);
-    g_a1 = row_sum<RL>(g_a1);
-    g_a2 = row_sum<RL>(g_a2);
+    acc.s6 = row_sum<RL>(acc.s6);
+    acc.s8 = row_sum<RL>(acc.s8);
+    acc.s10 = row_sum<RL>(acc.s10);
+    acc.a1 = row_sum<RL>(acc.a1);
+    acc.a2 = row_sum<RL>(acc.a2);
   }
-  if (lane == 0) {
-    w.e2[row] = acc_e;
-    if (GRAD) {
-      w.dldcn[row] = acc_cn;
-      w.dldq[row] = acc_q;
-      grad_out[3 * row] = fx;
-      grad_out[3 * row + 1] = fy;
-      grad_out[3 * row + 2] = fz;
-    }
-    if (PG) {
-      w.rt4[row] = sqrt(r4i);
-      w.pg2[row * 5 + 0] = g_s6;
-      w.pg2[row * 5 + 1] = g_s8;
-      w.pg2[row * 5 + 2] = g_s10;
-      w.pg2[row * 5 + 3] = g_a1;
-      w.pg2[row * 5 + 4] = g_a2;
-    }
-  }
+  if (lane == 0) store_pair_atom<GRAD, PG>(w, row, A.r4, acc, grad_out);
 }
 
 // --------------------------------------------------------------------------
@@ -821,9 +857,21 @@ struct ZPair {
   double kc;   // d log sqrt|C6(0)| / d CN_c
 };
 
+#ifndef D4_ATM1_UNROLL
+#define D4_ATM1_UNROLL 2
+#endif
+#ifndef D4_ATM8_FULL
+#define D4_ATM8_FULL 0
+#endif
+#ifndef D4_ATM1_WAVES
+#define D4_ATM1_WAVES 1
+#endif
+#define D4_PRAGMA(x) _Pragma(#x)
+#define D4_UNROLL(n) D4_PRAGMA(unroll n)
+
 template <bool ENERGY, bool GRAD, bool PG>
-__global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S,
-                                              int64_t nblocks) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAVES))) void k_atm1(
+    Params p, Inputs in, Work w, int S, int64_t nblocks) {
   constexpr int NA = kAtm1Acc;
   constexpr double kFar = 1.0e6;  // position of padding third atoms (bohr)
   // [z][x block | y block] pair records. The row stride is 67 16-byte units
@@ -936,7 +984,7 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
       __syncthreads();
 
       double rE = 0.0, rD = 0.0, rF0 = 0.0, rF1 = 0.0, rF2 = 0.0;  // ring sums
-#pragma unroll 2
+      D4_UNROLL(D4_ATM1_UNROLL)
       for (int k = 0; k < kTile; ++k) {
         const int zl = (k + tx) & (kTile - 1);
         const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
@@ -1096,11 +1144,345 @@ __global__ __launch_bounds__(256) void k_atm1(Params p, Inputs in, Work w, int S
   }
 }
 
+// --------------------------------------------------------------------------
+// One-pass three-body kernel with 8 x 8 pair tiles, one wave per work-group
+// (molecules of up to kAtm1MaxN atoms). Same per-triple arithmetic and
+// partition as k_atm1; the smaller tiles waste fewer slots on the diagonal
+// tiles, on third atoms z <= y and past the end of the molecule.
+//
+// Work-group (m, s) takes every S-th pair tile (bx <= by) of 8-atom blocks of
+// molecule m, S = atm8_splits(n). Lane (tx, ty) = (lane & 7, lane >> 3) owns
+// the pair x = 8 bx + tx < y = 8 by + ty and visits the third atoms z > y in
+// chunks of 8. Inside a chunk lane tx of a row handles z = z0 + (tx ^ k) at
+// step k, and the running sums for z move to the lane that handles z at the
+// next step: lane L takes the sums of lane L ^ (k ^ (k + 1)), a swap inside
+// each quad (xor 1 or 3) or a mirror of the 8-lane half row (xor 7), single
+// DPP moves. After 8 steps lane tx holds its row's sums for z0 + (tx ^ 7).
+// Every sum is added in a fixed order (no atomics).
+// --------------------------------------------------------------------------
+
+// Lane L of each 8-lane group receives the value of lane L ^ d (d = 1, 3, 7).
+template <int D>
+__device__ __forceinline__ double xor_lanes8(double v) {
+  // quad_perm [1,0,3,2], quad_perm [3,2,1,0], row_half_mirror
+  constexpr int ctrl = D == 1 ? 0xB1 : (D == 3 ? 0x1B : 0x141);
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, ctrl, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, ctrl, 0xF, 0xF, false));
+}
+
+template <int D>
+__device__ __forceinline__ void ring_xor8(double& a, double& b, double& c, double& d, double& e,
+                                          bool energy, bool grad) {
+  if (energy) a = xor_lanes8<D>(a);
+  if (grad) {
+    b = xor_lanes8<D>(b);
+    c = xor_lanes8<D>(c);
+    d = xor_lanes8<D>(d);
+    e = xor_lanes8<D>(e);
+  }
+}
+
+template <bool ENERGY, bool GRAD, bool PG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_atm8(
+    Params p, Inputs in, Work w, int S, int64_t nblocks) {
+  constexpr int NA = kAtm1Acc;
+  constexpr int T8 = kAtm8;
+  constexpr double kFar = 1.0e6;  // position of padding third atoms (bohr)
+  // [z][x block | y block] pair records (t, p, kz, kc); the row stride of 33
+  // 16-byte units (odd) keeps the skewed reads of a row on distinct banks
+  constexpr int kRow = 66;
+  __shared__ __align__(16) double s_zc[T8 * kRow];
+  __shared__ __align__(16) double s_r[T8 * 6];  // z position, dL/dE_z
+  __shared__ double s_rt[T8];                   // sqrt(r4r2_z) (parameter gradients)
+  __shared__ double s_red[NA][64];
+  extern __shared__ double s_acc[];             // [NA][N] atom sums of this split
+
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  const int64_t logical = (b % 8) * (G / 8) + b / 8;  // consecutive blocks on one XCD
+  if (logical >= nblocks) return;
+  const int64_t m = logical / S;
+  const int s = (int)(logical - m * S);
+  const int64_t N = w.N;
+  const int n = w.natom[m];
+  const int Sm = atm8_splits(n);  // this molecule's split count (S is the batch maximum)
+  if (s >= Sm) return;
+  const int nb = (n + T8 - 1) / T8;
+  const int tiles = nb * (nb + 1) / 2;
+  const int lane = threadIdx.x, tx = lane & (T8 - 1), ty = lane >> 3;
+  const int64_t* Z = in.numbers + m * N;
+  const double* R = in.positions + m * N * 3;
+  const double* pt = w.pt + m * N * N;
+  const double* pp = w.pp + m * N * N;
+  const double* pk = w.pk + m * N * N;
+  const double c3sq = p.cut3sq, s9 = p.s9, alp = p.alp;
+  for (int e = lane; e < NA * n; e += 64) s_acc[e] = 0.0;
+  double g_s9 = 0.0, g_alp = 0.0, g_a1 = 0.0, g_a2 = 0.0;  // parameter gradients (x 6)
+  const double a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
+
+  for (int t = s; t < tiles; t += Sm) {
+    int by = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while (by * (by + 1) / 2 > t) --by;
+    while ((by + 1) * (by + 2) / 2 <= t) ++by;
+    const int bx = t - by * (by + 1) / 2;
+    const int x = bx * T8 + tx, y = by * T8 + ty;
+    const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
+    double rx0 = 0.0, rx1 = 0.0, rx2 = 0.0, ry0 = 0.0, ry1 = 0.0, ry2 = 0.0;
+    double a = 1.0, ia = 1.0, txy = 0.0, pxy = 0.0, pxy0 = 0.0, kxy = 0.0, kyx = 0.0;
+    double gx = 0.0, gy = 0.0, mxy = 0.0;
+    if (valid) {
+      rx0 = R[3 * x];
+      rx1 = R[3 * x + 1];
+      rx2 = R[3 * x + 2];
+      ry0 = R[3 * y];
+      ry1 = R[3 * y + 1];
+      ry2 = R[3 * y + 2];
+      const double d0 = rx0 - ry0, d1 = rx1 - ry1, d2 = rx2 - ry2;
+      a = d0 * d0 + d1 * d1 + d2 * d2;
+      ia = 1.0 / a;
+      txy = pt[(int64_t)x * N + y];
+      pxy0 = pp[(int64_t)x * N + y];
+      pxy = s9 * pxy0;  // s9 folded into the pair factor
+      mxy = a <= c3sq ? 1.0 : 0.0;
+      if (GRAD) {
+        kxy = pk[(int64_t)x * N + y];
+        kyx = pk[(int64_t)y * N + x];
+        gx = in.gE ? in.gE[m * N + x] : 1.0;
+        gy = in.gE ? in.gE[m * N + y] : 1.0;
+      }
+    }
+    const double dxy0 = rx0 - ry0, dxy1 = rx1 - ry1, dxy2 = rx2 - ry2;
+    double rx4 = 0.0, ry4 = 0.0, Axy = 0.0, Bxy = 0.0;
+    if (PG && valid) {
+      rx4 = w.rt4[m * N + x];
+      ry4 = w.rt4[m * N + y];
+      const double sig = sq3 * rx4 * ry4;
+      Bxy = 1.0 / fma(a1p, sig, a2p);
+      Axy = sig * Bxy;
+    }
+    double ex = 0.0, ey = 0.0, cx = 0.0, cy = 0.0;
+    double fx0 = 0.0, fx1 = 0.0, fx2 = 0.0, fy0 = 0.0, fy1 = 0.0, fy2 = 0.0;
+
+    // records of a chunk (8 z x 16 columns, two per lane) and the third-atom
+    // data (lanes 0..7), loaded one chunk ahead into registers
+    double vt[2], vp[2], vkz[2], vkc[2], vr[5];
+    auto fetch = [&](int z0) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int e = lane + 64 * r;
+        const int zl = e >> 4, c = e & 15;
+        const int z = z0 + zl;
+        const int col = c < T8 ? bx * T8 + c : by * T8 + c - T8;
+        vt[r] = vp[r] = vkz[r] = vkc[r] = 0.0;
+        if (z < n && col < n) {
+          const int64_t idx = (int64_t)z * N + col;
+          vt[r] = pt[idx];
+          vp[r] = pp[idx];
+          if (GRAD) {
+            vkz[r] = pk[idx];
+            vkc[r] = pk[(int64_t)col * N + z];
+          }
+        }
+      }
+      if (lane < T8) {
+        const int z = z0 + lane;
+        const bool ok = z < n && Z[z] != 0;
+        vr[0] = ok ? R[3 * z] : kFar;
+        vr[1] = ok ? R[3 * z + 1] : kFar;
+        vr[2] = ok ? R[3 * z + 2] : kFar;
+        vr[3] = (GRAD && ok) ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0;
+        vr[4] = (PG && ok) ? w.rt4[m * N + z] : 0.0;
+      }
+    };
+    fetch(by * T8);
+
+    for (int z0 = by * T8; z0 < n; z0 += T8) {
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int e = lane + 64 * r;
+        const int zl = e >> 4, c = e & 15;
+        double2* dst = reinterpret_cast<double2*>(s_zc + zl * kRow + 4 * c);
+        dst[0] = make_double2(vt[r], vp[r]);
+        dst[1] = make_double2(vkz[r], vkc[r]);
+      }
+      if (lane < T8) {
+        double* r = s_r + 6 * lane;
+        r[0] = vr[0];
+        r[1] = vr[1];
+        r[2] = vr[2];
+        r[3] = vr[3];
+        if (PG) s_rt[lane] = vr[4];
+      }
+      __syncthreads();
+      if (z0 + T8 < n) fetch(z0 + T8);  // in flight during this chunk
+
+      double rE = 0.0, rD = 0.0, rF0 = 0.0, rF1 = 0.0, rF2 = 0.0;  // ring sums
+#if D4_ATM8_FULL
+#pragma unroll
+#else
+#pragma unroll 2
+#endif
+      for (int k = 0; k < T8; ++k) {
+#if D4_ATM8_FULL
+        __builtin_amdgcn_sched_barrier(0);  // keep the steps apart (register pressure)
+#endif
+        const int zl = tx ^ k;
+        const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
+        const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
+        const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
+        const double2* px = reinterpret_cast<const double2*>(s_zc + zl * kRow + 4 * tx);
+        const double2* py = reinterpret_cast<const double2*>(s_zc + zl * kRow + 4 * (T8 + ty));
+        const double2 x01 = px[0], y01 = py[0];
+        const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
+        const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
+        // third atoms up to y, outside the molecule or padding carry no weight
+        const double on = (valid && z0 + zl > y) ? 1.0 : 0.0;
+        const double u0 = rx0 - rz.x, u1 = rx1 - rz.y, u2 = rx2 - rz.z;
+        const double v0 = ry0 - rz.x, v1 = ry1 - rz.y, v2 = ry2 - rz.z;
+        const double b2 = fma(u2, u2, fma(u1, u1, u0 * u0));
+        const double c2 = fma(v2, v2, fma(v1, v1, v0 * v0));
+        const double bb = on != 0.0 ? b2 : 1.0;  // r_xz^2
+        const double cc = on != 0.0 ? c2 : 1.0;  // r_yz^2
+        const double T = txy * x01.x * y01.x;
+        const double P = on * pxy * x01.y * y01.y;
+        const double P0 = PG ? on * pxy0 * x01.y * y01.y : 0.0;  // without s9
+        const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
+        const double sabc = X * Y * W3;
+        const double bc = bb * cc;
+        const double abc = a * bc;
+        const double opT = fma(6.0, T, 1.0);
+        const double q0 = fast_rcp(abc * opT);
+        const double f = abc * q0;     // 1 / (1 + 6 T)
+        const double iabc = opT * q0;  // 1 / (a b c)
+        const double G2 = 0.375 * iabc;
+        const double G2s = G2 * sabc;
+        const double g = 1.0 + G2s;
+        const double K = P * f;  // s9 is in P
+        const double e = K * g;
+        const double mxz = bb <= c3sq ? 1.0 : 0.0, myz = cc <= c3sq ? 1.0 : 0.0;
+        const double m2 = mxz * myz;
+        const double sx = fma(myz, mxy, m2), sy = fma(mxz, mxy, m2), sz = mxy * (mxz + myz);
+        if (ENERGY) {
+          ex = fma(sx, e, ex);
+          ey = fma(sy, e, ey);
+          rE = fma(sz, e, rE);
+        }
+        if (GRAD) {
+          const double W6 = fma(gx, sx, fma(gy, sy, rz.w * sz));
+          const double We = W6 * e;
+          const double H = fma(g, fma(alp * T, f, -1.5), -G2s);
+          // ds/da, ds/db, ds/dc from A = Y W3, B = X W3, C = X Y
+          const double A = Y * W3, Bq = X * W3, C = X * Y;
+          const double Ssum = A + Bq + C;
+          const double KW = K * W6;  // half of the dL/de-weighted factor; x2 at the end
+          const double al = KW * H, be = KW * G2;
+          const double ali = al * a * iabc;  // al / (b c)
+          const double ca = fma(al, ia, be * fma(-2.0, C, Ssum));
+          const double cb = fma(ali, cc, be * fma(-2.0, Bq, Ssum));
+          const double cg = fma(ali, bb, be * fma(-2.0, A, Ssum));
+          fx0 = fma(ca, dxy0, fma(cb, u0, fx0));
+          fx1 = fma(ca, dxy1, fma(cb, u1, fx1));
+          fx2 = fma(ca, dxy2, fma(cb, u2, fx2));
+          fy0 = fma(cg, v0, fma(-ca, dxy0, fy0));
+          fy1 = fma(cg, v1, fma(-ca, dxy1, fy1));
+          fy2 = fma(cg, v2, fma(-ca, dxy2, fy2));
+          rF0 = fma(-cb, u0, fma(-cg, v0, rF0));
+          rF1 = fma(-cb, u1, fma(-cg, v1, rF1));
+          rF2 = fma(-cb, u2, fma(-cg, v2, rF2));
+          cx = fma(We, kxy + x23.y, cx);
+          cy = fma(We, kyx + y23.y, cy);
+          rD = fma(We, x23.x + y23.x, rD);
+          if (PG) {
+            const double TdT = -6.0 * f * We * T;  // dL6/dT times T
+            g_s9 = fma(W6, P0 * f * g, g_s9);
+            g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
+            const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
+            const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
+            g_a1 = fma(TdT * alp / 3.0, Axy + sxz * bxz + syz * byz, g_a1);
+            g_a2 = fma(TdT * alp / 3.0, Bxy + bxz + byz, g_a2);
+          }
+        }
+        // hand the sums for z to the lane that takes z at the next step:
+        // k ^ (k + 1) is 1, 3, 1, 7, 1, 3, 1
+        if ((k & 1) == 0)
+          ring_xor8<1>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
+        else if (k == 3)
+          ring_xor8<7>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
+        else if (k != T8 - 1)
+          ring_xor8<3>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
+      }
+      // lane tx of every row now holds that row's sums for z0 + (tx ^ 7) (the
+      // gradient parts at half weight); add the 8 rows in order
+      s_red[0][lane] = rE;
+      s_red[1][lane] = 2.0 * rF0;
+      s_red[2][lane] = 2.0 * rF1;
+      s_red[3][lane] = 2.0 * rF2;
+      s_red[4][lane] = rD;
+      __syncthreads();
+      if (lane < T8 * NA) {
+        const int l = lane & (T8 - 1), c = lane >> 3;  // column l holds z0 + (l ^ 7)
+        const int z = z0 + (l ^ 7);
+        if ((c == 0 ? ENERGY : GRAD) && z < n) {
+          double sum = 0.0;
+          for (int r = 0; r < T8; ++r) sum += s_red[c][r * T8 + l];
+          s_acc[c * n + z] += sum;
+        }
+      }
+    }
+    // tile sums of the x atoms (over the rows) and the y atoms (over the columns)
+    __syncthreads();
+    s_red[0][lane] = ex;
+    s_red[1][lane] = 2.0 * fx0;
+    s_red[2][lane] = 2.0 * fx1;
+    s_red[3][lane] = 2.0 * fx2;
+    s_red[4][lane] = cx;
+    __syncthreads();
+    if (lane < T8 * NA) {
+      const int col = lane & (T8 - 1), c = lane >> 3;
+      if ((c == 0 ? ENERGY : GRAD) && bx * T8 + col < n) {
+        double sum = 0.0;
+        for (int r = 0; r < T8; ++r) sum += s_red[c][r * T8 + col];
+        s_acc[c * n + bx * T8 + col] += sum;
+      }
+    }
+    __syncthreads();
+    s_red[0][lane] = ey;
+    s_red[1][lane] = 2.0 * fy0;
+    s_red[2][lane] = 2.0 * fy1;
+    s_red[3][lane] = 2.0 * fy2;
+    s_red[4][lane] = cy;
+    __syncthreads();
+    if (lane < T8 * NA) {
+      const int row = lane & (T8 - 1), c = lane >> 3;
+      if ((c == 0 ? ENERGY : GRAD) && by * T8 + row < n) {
+        double sum = 0.0;
+        for (int q = 0; q < T8; ++q) sum += s_red[c][row * T8 + q];
+        s_acc[c * n + by * T8 + row] += sum;
+      }
+    }
+  }
+  __syncthreads();
+  double* part = w.apart + (m * (int64_t)S + s) * NA * N;
+  for (int e = lane; e < NA * n; e += 64) {
+    const int c = e / n, i = e - c * n;
+    part[c * N + i] = s_acc[e];
+  }
+  if (PG) {
+    const double sixth = 1.0 / 6.0;
+    const double v4[4] = {g_s9 * sixth, g_alp * sixth, g_a1 * sixth, g_a2 * sixth};
+    for (int k = 0; k < 4; ++k) {
+      const double v = wave_sum(v4[k]);
+      if (lane == 0) w.pg3[(m * (int64_t)S + s) * 4 + k] = v;
+    }
+  }
+}
+
 // Atom sums of the one-pass three-body kernel over its splits: energies
 // (with the two-body part), gradient and dL/dCN.
 template <bool ENERGY, bool GRAD>
-__global__ __launch_bounds__(256) void k_atm1_rows(Inputs in, Work w, double* energy_out,
-                                                   double* grad_out) {
+__global__ __launch_bounds__(256) void k_atm1_rows(Inputs in, Work w, int tile8,
+                                                   double* energy_out, double* grad_out) {
   const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (row >= w.B * w.N) return;
   const int64_t N = w.N;
@@ -1112,8 +1494,8 @@ __global__ __launch_bounds__(256) void k_atm1_rows(Inputs in, Work w, double* en
     if (ENERGY) energy_out[row] = 0.0;
     return;
   }
-  const int S = atm1_splits(n);
-  const int64_t stride = (int64_t)atm1_splits((int)N) * kAtm1Acc * N;
+  const int S = one_pass_splits(n, tile8 != 0);
+  const int64_t stride = (int64_t)one_pass_splits((int)N, tile8 != 0) * kAtm1Acc * N;
   const double* part = w.apart + m * stride + i;
   double acc[kAtm1Acc] = {0.0, 0.0, 0.0, 0.0, 0.0};
   for (int s = 0; s < S; ++s)
@@ -1242,7 +1624,7 @@ __global__ __launch_bounds__(256) void k_copy(const double* src, double* dst,
 // Parameter gradients: fixed-order sums of the two-body row partials and the
 // three-body work-group partials into dL/d(s6, s8, s10, s9, a1, a2, alp).
 __global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int one_pass,
-                                                      int64_t per_mol, double* out) {
+                                                      int tile8, int64_t per_mol, double* out) {
   __shared__ double red[2 * kWaves];
   const int tid = threadIdx.x;
   const int64_t N = w.N;
@@ -1260,7 +1642,7 @@ __global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int one
     const int k = (int)(q - m * per_mol);
     const int n = w.natom[m];
     const int nb = (n + kTile - 1) / kTile;
-    const int valid = one_pass ? atm1_splits(n) : nb * (nb + 1) / 2;
+    const int valid = one_pass ? one_pass_splits(n, tile8 != 0) : nb * (nb + 1) / 2;
     if (n == 0 || k >= valid) continue;
     const double* g = w.pg3 + q * 4;
     a[3] += g[0];
@@ -1344,8 +1726,27 @@ hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
   return hipGetLastError();
 }
 
+hipError_t launch_atm8(hipStream_t s, const Params& p, const Inputs& in, const Work& w,
+                       bool energy, bool grad) {
+  const int S = atm8_splits((int)w.N);
+  const int64_t nblocks = w.B * S;
+  const int64_t grid = (nblocks + 7) / 8 * 8;  // multiple of 8 for the XCD remap
+  const size_t shmem = sizeof(double) * kAtm1Acc * (size_t)w.N;
+  const dim3 g((unsigned)grid), blk(64);
+  if (energy && grad)
+    hipLaunchKernelGGL((k_atm8<true, true, false>), g, blk, shmem, s, p, in, w, S, nblocks);
+  else if (energy)
+    hipLaunchKernelGGL((k_atm8<true, false, false>), g, blk, shmem, s, p, in, w, S, nblocks);
+  else if (grad && p.pgrad)
+    hipLaunchKernelGGL((k_atm8<false, true, true>), g, blk, shmem, s, p, in, w, S, nblocks);
+  else if (grad)
+    hipLaunchKernelGGL((k_atm8<false, true, false>), g, blk, shmem, s, p, in, w, S, nblocks);
+  return hipGetLastError();
+}
+
 hipError_t launch_atm1(hipStream_t s, const Params& p, const Inputs& in, const Work& w,
                        bool energy, bool grad) {
+  if (one_pass_tile8(p, grad)) return launch_atm8(s, p, in, w, energy, grad);
   const int S = atm1_splits((int)w.N);
   const int64_t nblocks = w.B * S;
   const int64_t grid = (nblocks + 7) / 8 * 8;  // multiple of 8 for the XCD remap
@@ -1366,7 +1767,7 @@ hipError_t launch_atm1(hipStream_t s, const Params& p, const Inputs& in, const W
 
 hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
                       const Work& w, bool energy, bool grad) {
-  if (w.N <= kAtm1MaxN && !p.force_pairtiles) return launch_atm1(s, p, in, w, energy, grad);
+  if (one_pass(p, w)) return launch_atm1(s, p, in, w, energy, grad);
   const int nb = (int)((w.N + kTile - 1) / kTile);
   const int tiles = nb * (nb + 1) / 2;
   const int64_t ntiles = w.B * tiles;
@@ -1392,16 +1793,17 @@ hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
 hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, const Work& w,
                            bool energy, bool grad, double* energy_out,
                            double* grad_out) {
-  if (w.N <= kAtm1MaxN && !p.force_pairtiles) {
+  if (one_pass(p, w)) {
     const unsigned grid = (unsigned)((w.B * w.N + 255) / 256);
+    const int tile8 = one_pass_tile8(p, grad) ? 1 : 0;
     if (energy && grad)
-      hipLaunchKernelGGL((k_atm1_rows<true, true>), dim3(grid), dim3(256), 0, s, in, w,
+      hipLaunchKernelGGL((k_atm1_rows<true, true>), dim3(grid), dim3(256), 0, s, in, w, tile8,
                          energy_out, grad_out);
     else if (energy)
-      hipLaunchKernelGGL((k_atm1_rows<true, false>), dim3(grid), dim3(256), 0, s, in, w,
+      hipLaunchKernelGGL((k_atm1_rows<true, false>), dim3(grid), dim3(256), 0, s, in, w, tile8,
                          energy_out, grad_out);
     else if (grad)
-      hipLaunchKernelGGL((k_atm1_rows<false, true>), dim3(grid), dim3(256), 0, s, in, w,
+      hipLaunchKernelGGL((k_atm1_rows<false, true>), dim3(grid), dim3(256), 0, s, in, w, tile8,
                          energy_out, grad_out);
     return hipGetLastError();
   }
@@ -1432,10 +1834,11 @@ hipError_t launch_properties(hipStream_t s, const DevTables& t, const Inputs& in
 
 hipError_t launch_pgrad_reduce(hipStream_t s, const Params& p, const Inputs& in,
                                const Work& w, double* out) {
-  const bool one_pass = w.N <= kAtm1MaxN && !p.force_pairtiles;
+  const bool op = one_pass(p, w);
+  const bool tile8 = one_pass_tile8(p, true);
   const int64_t nb = n_groups(w.N);
-  const int64_t per_mol = one_pass ? atm1_splits((int)w.N) : nb * (nb + 1) / 2;
-  hipLaunchKernelGGL(k_pgrad_reduce, dim3(1), dim3(256), 0, s, in, w, one_pass ? 1 : 0,
+  const int64_t per_mol = op ? one_pass_splits((int)w.N, tile8) : nb * (nb + 1) / 2;
+  hipLaunchKernelGGL(k_pgrad_reduce, dim3(1), dim3(256), 0, s, in, w, op ? 1 : 0, tile8 ? 1 : 0,
                      per_mol, out);
   return hipGetLastError();
 }

@@ -200,3 +200,31 @@ def test_one_pass_three_body_matches_pair_tiles():
         ga = runtime.backward(numbers, positions, charge, one, ws1, grad_energy=gE)
         gb = runtime.backward(numbers, positions, charge, ref, ws2, grad_energy=gE)
         assert (ga - gb).abs().max().item() < 1e-12
+
+
+def test_one_pass_8x8_matches_16x16():
+    """The two one-pass three-body kernels (8 x 8 pair tiles for the gradient
+    passes, 16 x 16 for energies) against each other: fused energies and
+    gradient, a weighted split backward and the parameter gradients, over
+    sizes that cross both tile edges (2 to 250 atoms) and a short cutoff."""
+    sizes = (2, 3, 8, 9, 15, 17, 33, 64, 95, 250)
+    _say(f"one-pass 8x8 vs 16x16 tiles, n = {sizes}")
+    mols = [_mol(140 + k, n) for k, n in enumerate(sizes)]
+    numbers, positions, charge = synth.pack(mols, device="cuda")
+    for cut in (_cut(), d4.Cutoff(disp2=30.0, disp3=7.0, cn=25.0, cn_eeq=12.0)):
+        t8 = _kp(cut)
+        t16 = _kp(cut, atm16=True)
+        e1, g1, _ = runtime.energy_gradient(numbers, positions, charge, t8)
+        e2, g2, _ = runtime.energy_gradient(numbers, positions, charge, t16)
+        assert (e1 - e2).abs().max().item() < 1e-13
+        assert (g1 - g2).abs().max().item() < 1e-12
+        gE = torch.rand_like(e1)
+        ea, ws1 = runtime.forward(numbers, positions, charge, t8)
+        eb, ws2 = runtime.forward(numbers, positions, charge, t16)
+        assert (ea - eb).abs().max().item() < 1e-14  # both energy passes use 16 x 16
+        ga, pa = runtime.backward(numbers, positions, charge, t8, ws1, grad_energy=gE,
+                                  params=True)
+        gb, pb = runtime.backward(numbers, positions, charge, t16, ws2, grad_energy=gE,
+                                  params=True)
+        assert (ga - gb).abs().max().item() < 1e-12
+        assert torch.allclose(pa, pb, rtol=1e-11, atol=1e-14)
