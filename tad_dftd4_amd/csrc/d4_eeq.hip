@@ -63,8 +63,13 @@ __device__ __forceinline__ int chol_diag_block(P L, int kb, int nb) {
   for (int j = 0; j < kNB; ++j) {
     const double piv = lane_bcast(d[j], j);
     bad |= !(piv > 0.0);
-    const double s = piv > 0.0 ? sqrt(piv) : 1.0;
-    const double inv = 1.0 / s;
+    // 1 / sqrt(piv): hardware estimate and two Newton steps (the diagonal
+    // of the factor itself is never needed, only its reciprocal)
+    const double pv = piv > 0.0 ? piv : 1.0;
+    double inv = __builtin_amdgcn_rsq(pv);
+    inv = inv * fma(-0.5 * pv * inv, inv, 1.5);
+    inv = inv * fma(-0.5 * pv * inv, inv, 1.5);
+    const double s = pv * inv;
     invd[j] = inv;
     // Lanes above the diagonal (lane < j) carry junk from here on in d[j] and
     // in the columns they update; only lanes >= column are ever broadcast
@@ -377,17 +382,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
     __syncthreads();
     const int ne = n * (n - 1) / 2;
-    for (int e = tid; e < ne; e += 256) {
-      const int i = strict_row(e);
-      const int j = e - i * (i - 1) / 2;
-      const double ri = srad[i], rj = srad[j];
-      double v = 0.0;
-      if (ri != 0.0 && rj != 0.0) {
-        const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        v = erf(r / sqrt(ri * ri + rj * rj)) / r;
+    // four independent elements per thread and iteration (the erf chains
+    // overlap)
+    for (int e0 = tid; e0 < ne; e0 += 4 * 256) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = e0 + 256 * u;
+        if (e < ne) {
+          const int i = strict_row(e);
+          const int j = e - i * (i - 1) / 2;
+          const double ri = srad[i], rj = srad[j];
+          double v = 0.0;
+          if (ri != 0.0 && rj != 0.0) {
+            const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            v = erf(r / sqrt(ri * ri + rj * rj)) / r;
+          }
+          L[tri(i) + j] = v;
+        }
       }
-      L[tri(i) + j] = v;
     }
   } else {
     for (int i = wave; i < n; i += kWaves) {

@@ -1,9 +1,10 @@
 // HIP kernels (gfx950) for the DFT-D4 energy and gradient path.
 //
-// FP64 throughout. Wave64 layout: "row" kernels give one wavefront to one
-// atom row i of one molecule (lanes stride over partner atoms j and reduce
-// with shuffles); the three-body kernel gives one work-group to a 16 x 16
-// tile of atom pairs of one molecule and streams the third atom through LDS.
+// FP64 throughout. Wave64 layout: "row" kernels give 16, 32 or 64 lanes to
+// one atom row i of one molecule (lanes stride over partner atoms j and
+// reduce with shuffles); the three-body kernels give a work-group to tiles
+// of atom pairs of one molecule (16 x 16 per work-group, or 8 x 8 per wave)
+// and stream the third atom through LDS.
 // The EEQ charge solve and its adjoint live in d4_eeq.hip.
 //
 // All reductions have a fixed order that depends only on the molecule (not
@@ -100,104 +101,95 @@ __global__ __launch_bounds__(256) void k_cn(DevTables t, Params p, Inputs in,
 
 // --------------------------------------------------------------------------
 // Gaussian reference weights (model/d4.py:137-228) and their derivatives,
-// one thread per atom
+// eight lanes per atom: lane a owns reference slot a (slot 7 only pads)
 // --------------------------------------------------------------------------
+
+// A double moved between lanes by one DPP control (both halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false));
+}
+
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int kDppXor3 = 0x1B;        // quad_perm [3,2,1,0]: lane ^ 3
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror: lane ^ 7 within 8 lanes
+
+// Sum over each group of 8 lanes.
+__device__ __forceinline__ double sum_lanes8(double v) {
+  v += dpp_double<kDppXor1>(v);
+  v += dpp_double<kDppXor2>(v);
+  v += dpp_double<kDppHalfMirror>(v);  // the other quad of the group
+  return v;
+}
 
 __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
                                                  Inputs in) {
-  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (row >= w.B * w.N) return;
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t row = tid >> 3;
+  const int a = (int)(tid & 7);
+  if (row >= w.B * w.N) return;  // uniform over each group of 8 lanes
   const int64_t m = row / w.N;
   const int i = (int)(row - m * w.N);
   const int z = i < w.natom[m] ? (int)in.numbers[row] : 0;
-  double* gw = w.gw + row * kRefStride;
-  double* gw0 = w.gw0 + row * kRefStride;
-  double* dgw = w.dgw + row * kRefStride;
-  double* dgw0 = w.dgw0 + row * kRefStride;
-  double* dgwq = w.dgwq + row * kRefStride;
   const int nr = z != 0 ? t.nref[z] : 0;
   const double cn = z != 0 ? w.cn[row] : 0.0;
   const double q = z != 0 ? w.q[row] : 0.0;
+  const bool used = a < nr;
+  const double zeff = t.zeff[z];
+  const double gam = t.gam[z] * p.gc;
+  const double qref = used ? t.refq[z * kMaxRef + a] + zeff : 0.0;
+  const double zq = used ? zeta_fn(p.ga, gam, qref, q + zeff) : 0.0;
+  const double z0 = used ? zeta_fn(p.ga, gam, qref, zeff) : 0.0;
+  const double dzq = used ? dzeta_fn(p.ga, gam, qref, q + zeff) : 0.0;
+  const int64_t o = row * kRefStride + a;
   if (p.d4s) {
     // D4S: the Gaussian part depends on the partner atom and is formed in
     // the pair pass; keep the charge scaling zeta(q), zeta(0), dzeta/dq of
     // every reference (model/d4s.py:240-263)
-    const double zeff = t.zeff[z];
-    const double gam = t.gam[z] * p.gc;
-#pragma unroll
-    for (int a = 0; a < kRefStride; ++a) {
-      double zq = 0.0, z0 = 0.0, dz = 0.0;
-      if (a < nr) {
-        const double qref = t.refq[z * kMaxRef + a] + zeff;
-        zq = zeta_fn(p.ga, gam, qref, q + zeff);
-        z0 = zeta_fn(p.ga, gam, qref, zeff);
-        dz = dzeta_fn(p.ga, gam, qref, q + zeff);
-      }
-      gw[a] = zq;
-      gw0[a] = z0;
-      dgw[a] = 0.0;
-      dgw0[a] = 0.0;
-      dgwq[a] = dz;
-    }
+    w.gw[o] = zq;
+    w.gw0[o] = z0;
+    w.dgw[o] = 0.0;
+    w.dgw0[o] = 0.0;
+    w.dgwq[o] = dzq;
     return;
   }
-  double expw[kMaxRef], dexpw[kMaxRef];
-  double norm = 0.0, dnorm = 0.0;
-#pragma unroll
-  for (int a = 0; a < kMaxRef; ++a) {
-    double e = 0.0, de = 0.0;
-    if (a < nr) {
-      // exp(-wf k d^2) evaluated as exp(-d^2)^(wf k), the reference's order
-      // of operations (model/d4.py:174-188): identical rounding where the
-      // Gaussians approach underflow
-      const double d = cn - t.refcn[z * kMaxRef + a];
-      const double tmp = exp(-(d * d));
-      const int ng = t.ngw[z * kMaxRef + a];
-      for (int k = 1; k <= ng; ++k) {
-        const double kw = k * p.wf;
-        const double ek = pow(tmp, kw);
-        e += ek;
-        de -= 2.0 * kw * d * ek;
-      }
+  double e = 0.0, de = 0.0;
+  if (used) {
+    // exp(-wf k d^2) evaluated as exp(-d^2)^(wf k), the reference's order
+    // of operations (model/d4.py:174-188): identical rounding where the
+    // Gaussians approach underflow
+    const double d = cn - t.refcn[z * kMaxRef + a];
+    const double tmp = exp(-(d * d));
+    const int ng = t.ngw[z * kMaxRef + a];
+    for (int k = 1; k <= ng; ++k) {
+      const double kw = k * p.wf;
+      const double ek = pow(tmp, kw);
+      e += ek;
+      de -= 2.0 * kw * d * ek;
     }
-    expw[a] = e;
-    dexpw[a] = de;
-    norm += e;
-    dnorm += de;
   }
+  const double norm = sum_lanes8(e);
+  const double dnorm = sum_lanes8(de);
   const bool exceptional = !(norm > 0.0);
-  const double maxcn = z != 0 ? t.maxcn[z] : 0.0;
-  const double zeff = t.zeff[z];
-  const double gam = t.gam[z] * p.gc;
-#pragma unroll
-  for (int a = 0; a < kRefStride; ++a) {
-    double v = 0.0, v0 = 0.0, dv = 0.0, dv0 = 0.0, dq = 0.0;
-    if (a < nr) {
-      double g, dg;
-      if (exceptional) {
-        g = t.refcn[z * kMaxRef + a] == maxcn ? 1.0 : 0.0;
-        dg = 0.0;
-      } else {
-        // form the ratio first: expw * dnorm underflows when the Gaussians
-        // are far below 1 (CN several units from every reference CN)
-        g = expw[a] / norm;
-        dg = (dexpw[a] - g * dnorm) / norm;
-      }
-      const double qref = t.refq[z * kMaxRef + a] + zeff;
-      const double zq = zeta_fn(p.ga, gam, qref, q + zeff);
-      const double z0 = zeta_fn(p.ga, gam, qref, zeff);
-      v = zq * g;
-      v0 = z0 * g;
-      dv = zq * dg;
-      dv0 = z0 * dg;
-      dq = dzeta_fn(p.ga, gam, qref, q + zeff) * g;
+  double g = 0.0, dg = 0.0;
+  if (used) {
+    if (exceptional) {
+      g = t.refcn[z * kMaxRef + a] == t.maxcn[z] ? 1.0 : 0.0;
+    } else {
+      // form the ratio first: expw * dnorm underflows when the Gaussians
+      // are far below 1 (CN several units from every reference CN)
+      g = e / norm;
+      dg = (de - g * dnorm) / norm;
     }
-    gw[a] = v;
-    gw0[a] = v0;
-    dgw[a] = dv;
-    dgw0[a] = dv0;
-    dgwq[a] = dq;
   }
+  w.gw[o] = zq * g;
+  w.gw0[o] = z0 * g;
+  w.dgw[o] = zq * dg;
+  w.dgw0[o] = z0 * dg;
+  w.dgwq[o] = dzq * g;
 }
 
 // --------------------------------------------------------------------------
@@ -860,9 +852,6 @@ struct ZPair {
 #ifndef D4_ATM1_UNROLL
 #define D4_ATM1_UNROLL 2
 #endif
-#ifndef D4_ATM8_FULL
-#define D4_ATM8_FULL 0
-#endif
 #ifndef D4_ATM1_WAVES
 #define D4_ATM1_WAVES 1
 #endif
@@ -1164,11 +1153,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
 // Lane L of each 8-lane group receives the value of lane L ^ d (d = 1, 3, 7).
 template <int D>
 __device__ __forceinline__ double xor_lanes8(double v) {
-  // quad_perm [1,0,3,2], quad_perm [3,2,1,0], row_half_mirror
-  constexpr int ctrl = D == 1 ? 0xB1 : (D == 3 ? 0x1B : 0x141);
-  const int lo = __double2loint(v), hi = __double2hiint(v);
-  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, ctrl, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(lo, lo, ctrl, 0xF, 0xF, false));
+  return dpp_double<D == 1 ? kDppXor1 : (D == 3 ? kDppXor3 : kDppHalfMirror)>(v);
 }
 
 template <int D>
@@ -1318,21 +1303,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
       if (z0 + T8 < n) fetch(z0 + T8);  // in flight during this chunk
 
       double rE = 0.0, rD = 0.0, rF0 = 0.0, rF1 = 0.0, rF2 = 0.0;  // ring sums
-#if D4_ATM8_FULL
-#pragma unroll
-#else
 #pragma unroll 2
-#endif
       for (int k = 0; k < T8; ++k) {
-#if D4_ATM8_FULL
-        __builtin_amdgcn_sched_barrier(0);  // keep the steps apart (register pressure)
-#endif
         const int zl = tx ^ k;
         const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
         const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
         const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
-        const double2* px = reinterpret_cast<const double2*>(s_zc + zl * kRow + 4 * tx);
-        const double2* py = reinterpret_cast<const double2*>(s_zc + zl * kRow + 4 * (T8 + ty));
+        const double* zrow = s_zc + __umul24(zl, kRow);
+        const double2* px = reinterpret_cast<const double2*>(zrow + 4 * tx);
+        const double2* py = reinterpret_cast<const double2*>(zrow + 4 * (T8 + ty));
         const double2 x01 = px[0], y01 = py[0];
         const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
         const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
@@ -1404,13 +1383,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
           }
         }
         // hand the sums for z to the lane that takes z at the next step:
-        // k ^ (k + 1) is 1, 3, 1, 7, 1, 3, 1
+        // k ^ (k + 1) is 1, 3, 1, 7, 1, 3, 1 (only the last step of each
+        // group of four depends on the group)
         if ((k & 1) == 0)
           ring_xor8<1>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
+        else if ((k & 3) == 1)
+          ring_xor8<3>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
         else if (k == 3)
           ring_xor8<7>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
-        else if (k != T8 - 1)
-          ring_xor8<3>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
       }
       // lane tx of every row now holds that row's sums for z0 + (tx ^ 7) (the
       // gradient parts at half weight); add the 8 rows in order
@@ -1697,7 +1677,7 @@ hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
 
 hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
                           const Work& w, const Inputs& in) {
-  const unsigned grid = (unsigned)((w.B * w.N + 255) / 256);
+  const unsigned grid = (unsigned)((w.B * w.N * 8 + 255) / 256);
   hipLaunchKernelGGL(k_weights, dim3(grid), dim3(256), 0, s, t, p, w, in);
   return hipGetLastError();
 }
