@@ -4,7 +4,9 @@
 
 #include <string.h>
 
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "d4_internal.h"
@@ -14,11 +16,22 @@ struct d4_event_pair {
   hipEvent_t start, stop;
 };
 
+// Companion stream of one caller stream: the three-body side (weights and
+// pair factors at q = 0, the three-body kernel) runs on it while the caller's
+// stream solves the EEQ system and evaluates the two-body term; two events
+// fork and join it.
+struct d4_aux {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
 struct d4_context {
   int device;
   void* mem;
   d4::DevTables t;
   bool has_wfpair = false;  // D4S factors supplied at creation
+  std::mutex aux_mu;
+  std::unordered_map<hipStream_t, d4_aux> aux;
   // diagnostics: per-kernel event timing
   bool profile = false;
   std::vector<d4_event_pair> pending;
@@ -107,6 +120,7 @@ struct Call {
   d4::Params p;
   hipStream_t s;
   bool sorted;
+  d4_aux aux;  // companion stream (null until fork)
 };
 
 Call make_call(void* stream, int64_t nbatch, int64_t natoms,
@@ -179,14 +193,102 @@ hipError_t timed(d4_context* ctx, hipStream_t s, int slot, F&& fn) {
 
 #define D4_RUN(slot, expr, where) \
   D4_TRY(timed(ctx, c.s, slot, [&]() { return (expr); }), where)
+// the same on the companion stream (kernels launched on c.aux.s)
+#define D4_RUN_AUX(slot, expr, where) \
+  D4_TRY(timed(ctx, c.aux.s, slot, [&]() { return (expr); }), where)
 
-int run_front(d4_context* ctx, Call& c) {
+// Companion stream of the call's stream, created on first use (on the
+// context's device, default priority, not blocking on the null stream).
+hipError_t companion(d4_context* ctx, hipStream_t s, d4_aux* out) {
+  std::lock_guard<std::mutex> lock(ctx->aux_mu);
+  auto it = ctx->aux.find(s);
+  if (it != ctx->aux.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  int prev = 0;
+  hipError_t e = hipGetDevice(&prev);
+  if (e == hipSuccess) e = hipSetDevice(ctx->device);
+  d4_aux a;
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a.fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&a.join, hipEventDisableTiming);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) return e;
+  ctx->aux.emplace(s, a);
+  *out = a;
+  return hipSuccess;
+}
+
+// Start the companion stream after everything enqueued so far on c.s.
+hipError_t fork_aux(d4_context* ctx, Call& c) {
+  hipError_t e = companion(ctx, c.s, &c.aux);
+  if (e == hipSuccess) e = hipEventRecord(c.aux.fork, c.s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c.aux.s, c.aux.fork, 0);
+  return e;
+}
+
+// Make c.s wait for the work enqueued on the companion stream.
+hipError_t join_aux(Call& c) {
+  hipError_t e = hipEventRecord(c.aux.join, c.aux.s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c.s, c.aux.join, 0);
+  return e;
+}
+
+// Sorted order, extents and coordination numbers on the call's stream.
+int run_cn(d4_context* ctx, Call& c) {
   D4_RUN(D4_SLOT_PREP, enter_sorted(c, false), "spatial order");
   D4_RUN(D4_SLOT_PREP, d4::launch_prep(c.s, c.in, c.w), "prep");
   D4_RUN(D4_SLOT_CN, d4::launch_cn(c.s, ctx->t, c.p, c.in, c.w), "coordination numbers");
+  return D4_OK;
+}
+
+// Everything up to the charges and all reference weights on one stream
+// (properties).
+int run_front(d4_context* ctx, Call& c) {
+  int rc = run_cn(ctx, c);
+  if (rc != D4_OK) return rc;
   D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
-  D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in),
+  D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsBoth),
          "reference weights");
+  return D4_OK;
+}
+
+// The two halves of an evaluation after the coordination numbers:
+//   companion: weights at q = 0 -> three-body pair factors -> three-body term
+//   call's:    EEQ charges -> weights at q -> two-body term
+// joined before the per-atom sums. `eeq` is false in the backward pass
+// (the forward left charges and weights in the workspace).
+int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double* g_out) {
+  if (c.p.d4s) {
+    // D4S forms the pair weights inside the pair pass: one fused pass on the
+    // call's stream instead of two that would each form them
+    if (eeq) {
+      D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
+      D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsBoth),
+             "reference weights");
+    }
+    D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, grad, g_out, d4::kPairsBoth),
+           "pair pass");
+    D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, energy, grad), "three-body term");
+    return D4_OK;
+  }
+  D4_TRY(fork_aux(ctx, c), "fork companion stream");
+  if (eeq)
+    D4_RUN_AUX(D4_SLOT_WEIGHTS, d4::launch_weights(c.aux.s, ctx->t, c.p, c.w, c.in, d4::kPairsAtm),
+               "reference weights (q = 0)");
+  D4_RUN_AUX(D4_SLOT_PAIRS,
+             d4::launch_pairs(c.aux.s, ctx->t, c.p, c.in, c.w, grad, nullptr, d4::kPairsAtm),
+             "three-body pair factors");
+  D4_RUN_AUX(D4_SLOT_ATM, d4::launch_atm(c.aux.s, c.p, c.in, c.w, energy, grad), "three-body term");
+  if (eeq) {
+    D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
+    D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsTwoBody),
+           "reference weights");
+  }
+  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, grad, g_out, d4::kPairsTwoBody),
+         "two-body term");
+  D4_TRY(join_aux(c), "join companion stream");
   return D4_OK;
 }
 
@@ -281,6 +383,12 @@ int d4_context_destroy(d4_context* ctx) {
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(ctx->device);
+  for (auto& kv : ctx->aux) {
+    (void)hipStreamSynchronize(kv.second.s);
+    (void)hipEventDestroy(kv.second.fork);
+    (void)hipEventDestroy(kv.second.join);
+    (void)hipStreamDestroy(kv.second.s);
+  }
   hipError_t e = hipFree(ctx->mem);
   (void)hipSetDevice(prev);
   delete ctx;
@@ -306,10 +414,9 @@ int d4_forward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   if (energy_out == nullptr) return set_error(D4_ERR_INVALID, "energy_out is NULL");
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
                      params, workspace);
-  if ((rc = run_front(ctx, c)) != D4_OK) return rc;
+  if ((rc = run_cn(ctx, c)) != D4_OK) return rc;
   double* e_out = out_atoms(c, energy_out, c.w.sE);
-  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, false, nullptr), "pair pass");
-  D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, true, false), "three-body term");
+  if ((rc = run_split(ctx, c, true, true, false, nullptr)) != D4_OK) return rc;
   D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, true, false, e_out, nullptr),
          "energy rows");
   D4_TRY(scatter_atoms(c, c.w.sE, energy_out, 1), "scatter energies");
@@ -338,9 +445,7 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   c.p.pgrad = grad_params != nullptr;
   D4_RUN(D4_SLOT_PREP, enter_sorted(c, true), "spatial order");  // the forward's order
   double* g_out = out_atoms(c, grad_positions, c.w.sG);
-  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, g_out),
-         "pair pass (gradient)");
-  D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, false, true), "three-body gradient");
+  if ((rc = run_split(ctx, c, false, false, true, g_out)) != D4_OK) return rc;
   D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, false, true, nullptr, g_out),
          "three-body rows");
   D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
@@ -363,12 +468,10 @@ int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch, int64_t na
     return set_error(D4_ERR_INVALID, "energy_out and grad_positions must be non-NULL");
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
                      params, workspace);
-  if ((rc = run_front(ctx, c)) != D4_OK) return rc;
+  if ((rc = run_cn(ctx, c)) != D4_OK) return rc;
   double* e_out = out_atoms(c, energy_out, c.w.sE);
   double* g_out = out_atoms(c, grad_positions, c.w.sG);
-  D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, true, g_out),
-         "pair pass (gradient)");
-  D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, true, true), "three-body term");
+  if ((rc = run_split(ctx, c, true, true, true, g_out)) != D4_OK) return rc;
   D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, true, true, e_out, g_out),
          "three-body rows");
   D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");

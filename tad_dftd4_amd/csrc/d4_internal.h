@@ -240,11 +240,17 @@ hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
                      const Inputs& in, const Work& w);
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
                       const Inputs& in, const Work& w, bool keep_factor);
+// Parts of the weights and pair passes: the three-body side (weights and C6
+// at q = 0, independent of the EEQ charges), the two-body side (at the EEQ
+// charges) or both.
+constexpr int kPairsBoth = 0;
+constexpr int kPairsAtm = 1;
+constexpr int kPairsTwoBody = 2;
 hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
-                          const Work& w, const Inputs& in);
+                          const Work& w, const Inputs& in, int part);
 hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, bool grad,
-                        double* grad_out);
+                        double* grad_out, int part);
 hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
                       const Work& w, bool energy, bool grad);
 hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, const Work& w,

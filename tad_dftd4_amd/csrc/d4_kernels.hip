@@ -125,8 +125,11 @@ __device__ __forceinline__ double sum_lanes8(double v) {
   return v;
 }
 
+// part kPairsAtm writes the q = 0 weights (gw0, dgw0), kPairsTwoBody those at
+// the EEQ charges (gw, dgw, dgwq), kPairsBoth all of them.
 __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
-                                                 Inputs in) {
+                                                 Inputs in, int part) {
+  const bool atm = part != kPairsTwoBody, two = part != kPairsAtm;
   const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t row = tid >> 3;
   const int a = (int)(tid & 7);
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
   const int z = i < w.natom[m] ? (int)in.numbers[row] : 0;
   const int nr = z != 0 ? t.nref[z] : 0;
   const double cn = z != 0 ? w.cn[row] : 0.0;
-  const double q = z != 0 ? w.q[row] : 0.0;
+  const double q = (z != 0 && two) ? w.q[row] : 0.0;
   const bool used = a < nr;
   const double zeff = t.zeff[z];
   const double gam = t.gam[z] * p.gc;
@@ -149,11 +152,15 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
     // D4S: the Gaussian part depends on the partner atom and is formed in
     // the pair pass; keep the charge scaling zeta(q), zeta(0), dzeta/dq of
     // every reference (model/d4s.py:240-263)
-    w.gw[o] = zq;
-    w.gw0[o] = z0;
-    w.dgw[o] = 0.0;
-    w.dgw0[o] = 0.0;
-    w.dgwq[o] = dzq;
+    if (two) {
+      w.gw[o] = zq;
+      w.dgw[o] = 0.0;
+      w.dgwq[o] = dzq;
+    }
+    if (atm) {
+      w.gw0[o] = z0;
+      w.dgw0[o] = 0.0;
+    }
     return;
   }
   double e = 0.0, de = 0.0;
@@ -185,11 +192,15 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
       dg = (de - g * dnorm) / norm;
     }
   }
-  w.gw[o] = zq * g;
-  w.gw0[o] = z0 * g;
-  w.dgw[o] = zq * dg;
-  w.dgw0[o] = z0 * dg;
-  w.dgwq[o] = dzq * g;
+  if (two) {
+    w.gw[o] = zq * g;
+    w.dgw[o] = zq * dg;
+    w.dgwq[o] = dzq * g;
+  }
+  if (atm) {
+    w.gw0[o] = z0 * g;
+    w.dgw0[o] = z0 * dg;
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -254,8 +265,14 @@ __device__ __forceinline__ void pair_gauss(const DevTables& t, int z, int nr, do
 // rational-damped two-body term, the ATM pair factors; with GRAD also the
 // two-body gradient pieces. With S (D4S model) the weights of both atoms are
 // formed per pair from the charge scaling that k_weights left and the pair's
-// Gaussian weights.
+// Gaussian weights. PART selects the three-body pair factors (C6 at q = 0:
+// independent of the charges), the two-body term (C6 at the EEQ charges) or
+// both in one pass.
 // --------------------------------------------------------------------------
+
+constexpr int kPartBoth = 0;
+constexpr int kPartAtm = 1;  // t, p, d log sqrt|C6(0)| / d CN (and sqrt r4r2 for PG)
+constexpr int kPartE2 = 2;   // two-body energy, dL/dCN, dL/dq, forces, PG partials
 
 // Atom i of a pair (the atom whose sums are accumulated).
 struct PairAtom {
@@ -271,10 +288,11 @@ struct PairSums {
   double s6 = 0.0, s8 = 0.0, s10 = 0.0, a1 = 0.0, a2 = 0.0;
 };
 
-template <bool GRAD, bool S>
+template <bool GRAD, bool S, int PART>
 __device__ __forceinline__ void load_pair_atom(const DevTables& t, const Work& w,
                                                const Inputs& in, int64_t row, int zi,
                                                const double* R, int i, PairAtom& A) {
+  constexpr bool ATMF = PART != kPartE2, E2 = PART != kPartAtm;
   A.z = zi;
   A.nr = t.nref[zi];
   A.x = R[3 * i];
@@ -282,36 +300,35 @@ __device__ __forceinline__ void load_pair_atom(const DevTables& t, const Work& w
   A.zc = R[3 * i + 2];
   A.r4 = in.r4r2_in ? in.r4r2_in[row] : t.r4r2[zi];
   A.cn = S ? w.cn[row] : 0.0;
-  A.g = GRAD ? (in.gE ? in.gE[row] : 1.0) : 0.0;
+  A.g = (GRAD && E2) ? (in.gE ? in.gE[row] : 1.0) : 0.0;
 #pragma unroll
   for (int a = 0; a < kMaxRef; ++a) {
-    A.gw[a] = w.gw[row * kRefStride + a];
-    A.gw0[a] = w.gw0[row * kRefStride + a];
-    if (GRAD) {
-      A.dgw[a] = S ? 0.0 : w.dgw[row * kRefStride + a];
-      A.dgw0[a] = S ? 0.0 : w.dgw0[row * kRefStride + a];
-      A.dgwq[a] = w.dgwq[row * kRefStride + a];
-    }
+    A.gw[a] = E2 ? w.gw[row * kRefStride + a] : 0.0;
+    A.gw0[a] = ATMF ? w.gw0[row * kRefStride + a] : 0.0;
+    A.dgw[a] = (GRAD && E2 && !S) ? w.dgw[row * kRefStride + a] : 0.0;
+    A.dgw0[a] = (GRAD && ATMF && !S) ? w.dgw0[row * kRefStride + a] : 0.0;
+    A.dgwq[a] = (GRAD && E2) ? w.dgwq[row * kRefStride + a] : 0.0;
   }
 }
 
 // One pair (i, j): accumulates atom i's two-body energy, dL/dCN_i, dL/dq_i,
 // forces and parameter partials, and returns the ATM pair factors
 // tf = (R0/r)^(alp/3), pf = sqrt|C6(0)|/r^3 and pk = d log sqrt|C6(0)| / d CN_i.
-template <bool GRAD, bool PG, bool S>
+template <bool GRAD, bool PG, bool S, int PART>
 __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, const PairAtom& A,
                                           int zj, double xj, double yj, double zzj, double r4j,
                                           double cnj, double gj, const double* gwjp,
                                           const double* gw0jp, PairSums& acc, double& tf,
                                           double& pf, double& pk) {
+  constexpr bool ATMF = PART != kPartE2, E2 = PART != kPartAtm;
   const double dx = A.x - xj, dy = A.y - yj, dz = A.zc - zzj;
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double r = sqrt(r2);
   double gwj[kMaxRef], gw0j[kMaxRef];
 #pragma unroll
   for (int b = 0; b < kMaxRef; ++b) {
-    gwj[b] = gwjp[b];
-    gw0j[b] = gw0jp[b];
+    gwj[b] = E2 ? gwjp[b] : 0.0;
+    gw0j[b] = ATMF ? gw0jp[b] : 0.0;
   }
   // D4S: Gaussian weights of both atoms for this pair; atom n's weights for
   // partner m use wf(Z_n, Z_m) (model/d4s.py:196). The weights of atom i
@@ -336,30 +353,38 @@ __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, c
 #pragma unroll
     for (int b = 0; b < kMaxRef; ++b) {
       const double rc = blk[a * kMaxRef + b];
-      va += rc * gwj[b];
-      v0a += rc * gw0j[b];
+      if (E2) va += rc * gwj[b];
+      if (ATMF) v0a += rc * gw0j[b];
     }
     if (S) {
-      c6 += A.gw[a] * g[a] * va;
-      c60 += A.gw0[a] * g[a] * v0a;
-      if (GRAD) {
+      if (E2) c6 += A.gw[a] * g[a] * va;
+      if (ATMF) c60 += A.gw0[a] * g[a] * v0a;
+      if (GRAD && E2) {
         dc6cn += A.gw[a] * dg[a] * va;
         dc6q += A.dgwq[a] * g[a] * va;
-        dc60cn += A.gw0[a] * dg[a] * v0a;
       }
+      if (GRAD && ATMF) dc60cn += A.gw0[a] * dg[a] * v0a;
     } else {
-      c6 += A.gw[a] * va;
-      c60 += A.gw0[a] * v0a;
-      if (GRAD) {
+      if (E2) c6 += A.gw[a] * va;
+      if (ATMF) c60 += A.gw0[a] * v0a;
+      if (GRAD && E2) {
         dc6cn += A.dgw[a] * va;
         dc6q += A.dgwq[a] * va;
-        dc60cn += A.dgw0[a] * v0a;
       }
+      if (GRAD && ATMF) dc60cn += A.dgw0[a] * v0a;
     }
   }
   // rational damping, radii = 3 r4r2_i r4r2_j (twobody.py:147-157)
   const double qq = 3.0 * A.r4 * r4j;
   const double r0 = p.a1 * sqrt(qq) + p.a2;
+  tf = pf = pk = 0.0;
+  if (ATMF) {
+    // ATM pair factors (threebody.py:106-145, 241-256, 311-321)
+    tf = pow(r0 / r, p.alp / 3.0);
+    pf = sqrt(fabs(c60)) / (r2 * r);
+    if (GRAD) pk = c60 != 0.0 ? 0.5 * dc60cn / c60 : 0.0;
+  }
+  if (!E2) return;
   const double r6 = r2 * r2 * r2, r8 = r6 * r2;
   const double r06 = r0 * r0 * r0 * r0 * r0 * r0, r08 = r06 * r0 * r0;
   const double t6 = 1.0 / (r6 + r06);
@@ -387,31 +412,27 @@ __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, c
     acc.a1 = fma(ce * dr0, sqrt(qq), acc.a1);
     acc.a2 = fma(ce, dr0, acc.a2);
   }
-  // ATM pair factors (threebody.py:106-145, 241-256, 311-321)
-  tf = pow(r0 / r, p.alp / 3.0);
-  pf = sqrt(fabs(c60)) / (r2 * r);
-  pk = 0.0;
-  if (GRAD) {
-    pk = c60 != 0.0 ? 0.5 * dc60cn / c60 : 0.0;
-    if (in2) {
-      const double wij = A.g + gj;
-      acc.cn += wij * de_dc6 * dc6cn;
-      acc.q += wij * de_dc6 * dc6q;
-      // d/dr of 1/(r^n + r0^n) = -n r^(n-1) t^2
-      double dedr = p.s6 * (-6.0 * r6 / r) * t6 * t6 + p.s8 * qq * (-8.0 * r8 / r) * t8 * t8;
-      if (p.s10on) dedr += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r10 / r) * t10 * t10;
-      dedr *= -0.5 * c6;
-      const double coef = wij * dedr / r;
-      acc.fx += coef * dx;
-      acc.fy += coef * dy;
-      acc.fz += coef * dz;
-    }
+  if (GRAD && in2) {
+    const double wij = A.g + gj;
+    acc.cn += wij * de_dc6 * dc6cn;
+    acc.q += wij * de_dc6 * dc6q;
+    // d/dr of 1/(r^n + r0^n) = -n r^(n-1) t^2
+    double dedr = p.s6 * (-6.0 * r6 / r) * t6 * t6 + p.s8 * qq * (-8.0 * r8 / r) * t8 * t8;
+    if (p.s10on) dedr += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r10 / r) * t10 * t10;
+    dedr *= -0.5 * c6;
+    const double coef = wij * dedr / r;
+    acc.fx += coef * dx;
+    acc.fy += coef * dy;
+    acc.fz += coef * dz;
   }
 }
 
-template <bool GRAD, bool PG>
+template <bool GRAD, bool PG, int PART>
 __device__ __forceinline__ void store_pair_atom(const Work& w, int64_t row, double r4,
                                                 const PairSums& acc, double* grad_out) {
+  constexpr bool ATMF = PART != kPartE2, E2 = PART != kPartAtm;
+  if (PG && ATMF) w.rt4[row] = sqrt(r4);
+  if (!E2) return;
   w.e2[row] = acc.e;
   if (GRAD) {
     w.dldcn[row] = acc.cn;
@@ -421,7 +442,6 @@ __device__ __forceinline__ void store_pair_atom(const Work& w, int64_t row, doub
     grad_out[3 * row + 2] = acc.fz;
   }
   if (PG) {
-    w.rt4[row] = sqrt(r4);
     w.pg2[row * 5 + 0] = acc.s6;
     w.pg2[row * 5 + 1] = acc.s8;
     w.pg2[row * 5 + 2] = acc.s10;
@@ -432,9 +452,10 @@ __device__ __forceinline__ void store_pair_atom(const Work& w, int64_t row, doub
 
 // RL lanes per atom row i walk the partners j; pt[i][j], pp[i][j] and
 // pk[i][j] are written along the row.
-template <bool GRAD, bool PG, bool S, int RL>
+template <bool GRAD, bool PG, bool S, int RL, int PART>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pairs(
     DevTables t, Params p, Inputs in, Work w, double* grad_out) {
+  constexpr bool ATMF = PART != kPartE2, E2 = PART != kPartAtm;
   const int lane = threadIdx.x & (RL - 1);
   const int64_t row = (int64_t)blockIdx.x * (256 / RL) + threadIdx.x / RL;
   if (row >= w.B * w.N) return;
@@ -449,16 +470,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   double* pkr = w.pk + row * N;
   const int zi = i < n ? (int)Z[i] : 0;
   if (zi == 0) {
-    for (int j = lane; j < n; j += RL) {
-      ptr[j] = 0.0;
-      ppr[j] = 0.0;
-      if (GRAD) pkr[j] = 0.0;
+    if (ATMF) {
+      for (int j = lane; j < n; j += RL) {
+        ptr[j] = 0.0;
+        ppr[j] = 0.0;
+        if (GRAD) pkr[j] = 0.0;
+      }
     }
-    if (lane == 0) store_pair_atom<GRAD, PG>(w, row, 0.0, PairSums{}, grad_out);
+    if (lane == 0) store_pair_atom<GRAD, PG, PART>(w, row, 0.0, PairSums{}, grad_out);
     return;
   }
   PairAtom A;
-  load_pair_atom<GRAD, S>(t, w, in, row, zi, R, i, A);
+  load_pair_atom<GRAD, S, PART>(t, w, in, row, zi, R, i, A);
   PairSums acc;
   // the partner's Z is loaded one iteration ahead: the table reads of a pair
   // (reference C6 block, r4r2) depend on it
@@ -467,38 +490,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int zj = znext;
     znext = j + RL < n ? (int)Z[j + RL] : 0;
     if (j == i || zj == 0) {
-      ptr[j] = 0.0;
-      ppr[j] = 0.0;
-      if (GRAD) pkr[j] = 0.0;
+      if (ATMF) {
+        ptr[j] = 0.0;
+        ppr[j] = 0.0;
+        if (GRAD) pkr[j] = 0.0;
+      }
       continue;
     }
     const int64_t rj = m * N + j;
     const double r4j = in.r4r2_in ? in.r4r2_in[rj] : t.r4r2[zj];
     const double cnj = S ? w.cn[rj] : 0.0;
-    const double gj = GRAD ? (in.gE ? in.gE[rj] : 1.0) : 0.0;
+    const double gj = (GRAD && E2) ? (in.gE ? in.gE[rj] : 1.0) : 0.0;
     double tf, pf, pk;
-    pair_eval<GRAD, PG, S>(t, p, A, zj, R[3 * j], R[3 * j + 1], R[3 * j + 2], r4j, cnj, gj,
-                           w.gw + rj * kRefStride, w.gw0 + rj * kRefStride, acc, tf, pf, pk);
-    ptr[j] = tf;
-    ppr[j] = pf;
-    if (GRAD) pkr[j] = pk;
+    pair_eval<GRAD, PG, S, PART>(t, p, A, zj, R[3 * j], R[3 * j + 1], R[3 * j + 2], r4j, cnj,
+                                 gj, w.gw + rj * kRefStride, w.gw0 + rj * kRefStride, acc, tf,
+                                 pf, pk);
+    if (ATMF) {
+      ptr[j] = tf;
+      ppr[j] = pf;
+      if (GRAD) pkr[j] = pk;
+    }
   }
-  acc.e = row_sum<RL>(acc.e);
-  if (GRAD) {
-    acc.cn = row_sum<RL>(acc.cn);
-    acc.q = row_sum<RL>(acc.q);
-    acc.fx = row_sum<RL>(acc.fx);
-    acc.fy = row_sum<RL>(acc.fy);
-    acc.fz = row_sum<RL>(acc.fz);
+  if (E2) {
+    acc.e = row_sum<RL>(acc.e);
+    if (GRAD) {
+      acc.cn = row_sum<RL>(acc.cn);
+      acc.q = row_sum<RL>(acc.q);
+      acc.fx = row_sum<RL>(acc.fx);
+      acc.fy = row_sum<RL>(acc.fy);
+      acc.fz = row_sum<RL>(acc.fz);
+    }
+    if (PG) {
+      acc.s6 = row_sum<RL>(acc.s6);
+      acc.s8 = row_sum<RL>(acc.s8);
+      acc.s10 = row_sum<RL>(acc.s10);
+      acc.a1 = row_sum<RL>(acc.a1);
+      acc.a2 = row_sum<RL>(acc.a2);
+    }
   }
-  if (PG) {
-    acc.s6 = row_sum<RL>(acc.s6);
-    acc.s8 = row_sum<RL>(acc.s8);
-    acc.s10 = row_sum<RL>(acc.s10);
-    acc.a1 = row_sum<RL>(acc.a1);
-    acc.a2 = row_sum<RL>(acc.a2);
-  }
-  if (lane == 0) store_pair_atom<GRAD, PG>(w, row, A.r4, acc, grad_out);
+  if (lane == 0) store_pair_atom<GRAD, PG, PART>(w, row, A.r4, acc, grad_out);
 }
 
 // --------------------------------------------------------------------------
@@ -1676,33 +1706,44 @@ hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
 }
 
 hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
-                          const Work& w, const Inputs& in) {
+                          const Work& w, const Inputs& in, int part) {
   const unsigned grid = (unsigned)((w.B * w.N * 8 + 255) / 256);
-  hipLaunchKernelGGL(k_weights, dim3(grid), dim3(256), 0, s, t, p, w, in);
+  hipLaunchKernelGGL(k_weights, dim3(grid), dim3(256), 0, s, t, p, w, in, part);
   return hipGetLastError();
 }
 
-template <bool S>
+template <bool S, int PART>
 void launch_pairs_model(hipStream_t s, const DevTables& t, const Params& p, const Inputs& in,
                         const Work& w, bool grad, double* grad_out) {
   if (grad && p.pgrad)
-    D4_LAUNCH_ROWS((k_pairs<true, true, S, 16>), (k_pairs<true, true, S, 32>),
-                   (k_pairs<true, true, S, 64>), s, w, t, p, in, w, grad_out);
+    D4_LAUNCH_ROWS((k_pairs<true, true, S, 16, PART>), (k_pairs<true, true, S, 32, PART>),
+                   (k_pairs<true, true, S, 64, PART>), s, w, t, p, in, w, grad_out);
   else if (grad)
-    D4_LAUNCH_ROWS((k_pairs<true, false, S, 16>), (k_pairs<true, false, S, 32>),
-                   (k_pairs<true, false, S, 64>), s, w, t, p, in, w, grad_out);
+    D4_LAUNCH_ROWS((k_pairs<true, false, S, 16, PART>), (k_pairs<true, false, S, 32, PART>),
+                   (k_pairs<true, false, S, 64, PART>), s, w, t, p, in, w, grad_out);
   else
-    D4_LAUNCH_ROWS((k_pairs<false, false, S, 16>), (k_pairs<false, false, S, 32>),
-                   (k_pairs<false, false, S, 64>), s, w, t, p, in, w, grad_out);
+    D4_LAUNCH_ROWS((k_pairs<false, false, S, 16, PART>), (k_pairs<false, false, S, 32, PART>),
+                   (k_pairs<false, false, S, 64, PART>), s, w, t, p, in, w, grad_out);
+}
+
+template <int PART>
+void launch_pairs_part(hipStream_t s, const DevTables& t, const Params& p, const Inputs& in,
+                       const Work& w, bool grad, double* grad_out) {
+  if (p.d4s)
+    launch_pairs_model<true, PART>(s, t, p, in, w, grad, grad_out);
+  else
+    launch_pairs_model<false, PART>(s, t, p, in, w, grad, grad_out);
 }
 
 hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, bool grad,
-                        double* grad_out) {
-  if (p.d4s)
-    launch_pairs_model<true>(s, t, p, in, w, grad, grad_out);
+                        double* grad_out, int part) {
+  if (part == kPairsAtm)
+    launch_pairs_part<kPartAtm>(s, t, p, in, w, grad, grad_out);
+  else if (part == kPairsTwoBody)
+    launch_pairs_part<kPartE2>(s, t, p, in, w, grad, grad_out);
   else
-    launch_pairs_model<false>(s, t, p, in, w, grad, grad_out);
+    launch_pairs_part<kPartBoth>(s, t, p, in, w, grad, grad_out);
   return hipGetLastError();
 }
 

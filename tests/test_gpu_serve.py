@@ -57,3 +57,30 @@ def test_stream_alone_vs_mix():
                          {k: torch.tensor(v, dtype=torch.float64) for k, v in PARAM.items()},
                          cutoff={"disp2": 50.0, "disp3": 25.0, "cn_eeq": 25.0})
         assert np.abs(by_id[req.rid].energy - eo.numpy()).max() < 1e-10
+
+
+def test_two_streams_share_a_context():
+    """Calls on two torch streams at once (each gets its own companion
+    stream for the three-body side) give the results of calls made one after
+    the other."""
+    print("[case] two streams, one context", flush=True)
+    cut = Cutoff(disp2=50.0, disp3=25.0, cn_eeq=25.0)
+    kp = kernel_params(PARAM, cut, None)
+    reqs = _requests(40, seed=11)
+    mols = [synth.Molecule(r.numbers, r.positions, r.charge) for r in reqs]
+    a = synth.pack(mols[:20], device="cuda")
+    b = synth.pack(mols[20:], device="cuda")
+    ea, ga, _ = runtime.energy_gradient(*a, kp)
+    eb, _ = runtime.forward(*b, kp)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    outs = []
+    for _ in range(3):
+        with torch.cuda.stream(s1):
+            e1, g1, _ = runtime.energy_gradient(*a, kp)
+        with torch.cuda.stream(s2):
+            e2, _ = runtime.forward(*b, kp)
+        outs.append((e1, g1, e2))
+    torch.cuda.synchronize()
+    for e1, g1, e2 in outs:
+        assert torch.equal(e1, ea) and torch.equal(g1, ga) and torch.equal(e2, eb)
