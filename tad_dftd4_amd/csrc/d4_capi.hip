@@ -258,7 +258,9 @@ int run_front(d4_context* ctx, Call& c) {
 //   companion: weights at q = 0 -> three-body pair factors -> three-body term
 //   call's:    EEQ charges -> weights at q -> two-body term
 // joined before the per-atom sums. `eeq` is false in the backward pass
-// (the forward left charges and weights in the workspace).
+// (the forward left charges and weights in the workspace). With `grad` the
+// EEQ adjoint (which needs only dL/dq of the two-body term) also runs before
+// the join.
 int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double* g_out) {
   if (c.p.d4s) {
     // D4S forms the pair weights inside the pair pass: one fused pass on the
@@ -270,6 +272,8 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
     }
     D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, grad, g_out, d4::kPairsBoth),
            "pair pass");
+    if (grad)
+      D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
     D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, energy, grad), "three-body term");
     return D4_OK;
   }
@@ -288,6 +292,8 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
   }
   D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, grad, g_out, d4::kPairsTwoBody),
          "two-body term");
+  if (grad)
+    D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
   D4_TRY(join_aux(c), "join companion stream");
   return D4_OK;
 }
@@ -448,7 +454,6 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   if ((rc = run_split(ctx, c, false, false, true, g_out)) != D4_OK) return rc;
   D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, false, true, nullptr, g_out),
          "three-body rows");
-  D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
   D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, g_out), "CN/EEQ chain");
   D4_TRY(scatter_atoms(c, c.w.sG, grad_positions, 3), "scatter gradient");
   if (grad_params)
@@ -474,7 +479,6 @@ int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch, int64_t na
   if ((rc = run_split(ctx, c, true, true, true, g_out)) != D4_OK) return rc;
   D4_RUN(D4_SLOT_ATM_ROWS, d4::launch_atm_rows(c.s, c.p, c.in, c.w, true, true, e_out, g_out),
          "three-body rows");
-  D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
   D4_RUN(D4_SLOT_CHAIN, d4::launch_chain(c.s, ctx->t, c.p, c.in, c.w, g_out), "CN/EEQ chain");
   D4_TRY(scatter_atoms(c, c.w.sE, energy_out, 1), "scatter energies");
   D4_TRY(scatter_atoms(c, c.w.sG, grad_positions, 3), "scatter gradient");
