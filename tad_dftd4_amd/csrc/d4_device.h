@@ -34,6 +34,12 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
 __device__ __forceinline__ int wave_or(int v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v |= __shfl_xor(v, off, 64);
