@@ -927,6 +927,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
   for (int e = tid; e < n * NA; e += 256) (&s_acc[0][0])[e] = 0.0;
   // parameter gradients (times 6): s9, alp, a1, a2
   double g_s9 = 0.0, g_alp = 0.0, g_a1 = 0.0, g_a2 = 0.0;
+  // molecules whose bounding box fits in the cutoff sphere have every pair
+  // within the cutoff: all screening weights are 2 and their masks drop out
+  bool full;
+  {
+    double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
+    for (int i = tid & 63; i < n; i += 64) {
+      if (Z[i] == 0) continue;
+      lo0 = fmin(lo0, R[3 * i]);
+      hi0 = fmax(hi0, R[3 * i]);
+      lo1 = fmin(lo1, R[3 * i + 1]);
+      hi1 = fmax(hi1, R[3 * i + 1]);
+      lo2 = fmin(lo2, R[3 * i + 2]);
+      hi2 = fmax(hi2, R[3 * i + 2]);
+    }
+    const double d0 = wave_max(hi0) + wave_max(-lo0), d1 = wave_max(hi1) + wave_max(-lo1),
+                 d2 = wave_max(hi2) + wave_max(-lo2);
+    full = d0 * d0 + d1 * d1 + d2 * d2 <= c3sq;  // the same in every wave
+  }
   const double a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
 
   for (int t = s; t < tiles; t += Sm) {
@@ -936,7 +954,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
     const int bx = t - by * (by + 1) / 2;
     const int x = bx * kTile + tx, y = by * kTile + ty;
     const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
-    double rx0 = 0.0, rx1 = 0.0, rx2 = 0.0, ry0 = 0.0, ry1 = 0.0, ry2 = 0.0;
+    // lanes without a pair sit far from every atom (no zero distance; their
+    // pair factor is zero)
+    double rx0 = -kFar, rx1 = -kFar, rx2 = -kFar, ry0 = -kFar, ry1 = -kFar, ry2 = -kFar;
     double a = 1.0, ia = 1.0, txy = 0.0, pxy = 0.0, pxy0 = 0.0, kxy = 0.0, kyx = 0.0;
     double gx = 0.0, gy = 0.0, mxy = 0.0;
     if (valid) {
@@ -969,7 +989,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
       Bxy = 1.0 / fma(a1p, sig, a2p);
       Axy = sig * Bxy;
     }
-    double ex = 0.0, ey = 0.0, cx = 0.0, cy = 0.0;
+    const double gxy = gx + gy;
+    double ex = 0.0, ey = 0.0, cx = 0.0, cy = 0.0, sca = 0.0;
     double fx0 = 0.0, fx1 = 0.0, fx2 = 0.0, fy0 = 0.0, fy1 = 0.0, fy2 = 0.0;
 
     for (int z0 = by * kTile; z0 < n; z0 += kTile) {
@@ -1005,93 +1026,131 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
       __syncthreads();
 
       double rE = 0.0, rD = 0.0, rF0 = 0.0, rF1 = 0.0, rF2 = 0.0;  // ring sums
-      D4_UNROLL(D4_ATM1_UNROLL)
-      for (int k = 0; k < kTile; ++k) {
-        const int zl = (k + tx) & (kTile - 1);
-        const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
-        const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
-        const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
-        const double2* px = reinterpret_cast<const double2*>(s_zc + zl * kZcRow + 4 * tx);
-        const double2* py = reinterpret_cast<const double2*>(s_zc + zl * kZcRow + 4 * (kTile + ty));
-        const double2 x01 = px[0], y01 = py[0];
-        const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
-        const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
-        const ZPair zx = {x01.x, x01.y, x23.x, x23.y};
-        const ZPair zy = {y01.x, y01.y, y23.x, y23.y};
-        // third atoms up to y, outside the molecule or padding carry no weight
-        const double on = (valid && z0 + zl > y) ? 1.0 : 0.0;
-        const double u0 = rx0 - rz.x, u1 = rx1 - rz.y, u2 = rx2 - rz.z;
-        const double v0 = ry0 - rz.x, v1 = ry1 - rz.y, v2 = ry2 - rz.z;
-        const double b2 = fma(u2, u2, fma(u1, u1, u0 * u0));
-        const double c2 = fma(v2, v2, fma(v1, v1, v0 * v0));
-        const double bb = on != 0.0 ? b2 : 1.0;  // r_xz^2
-        const double cc = on != 0.0 ? c2 : 1.0;  // r_yz^2
-        const double T = txy * zx.t * zy.t;
-        const double P = on * pxy * zx.p * zy.p;
-        const double P0 = PG ? on * pxy0 * zx.p * zy.p : 0.0;  // without s9
-        const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
-        const double sabc = X * Y * W3;
-        const double bc = bb * cc;
-        const double abc = a * bc;
-        const double opT = fma(6.0, T, 1.0);
-        const double q0 = fast_rcp(abc * opT);
-        const double f = abc * q0;     // 1 / (1 + 6 T)
-        const double iabc = opT * q0;  // 1 / (a b c)
-        const double G2 = 0.375 * iabc;
-        const double g = fma(G2, sabc, 1.0);
-        const double K = P * f;  // s9 is in P
-        const double e = K * g;
-        const double mxz = bb <= c3sq ? 1.0 : 0.0, myz = cc <= c3sq ? 1.0 : 0.0;
-        const double m2 = mxz * myz;
-        const double sx = fma(myz, mxy, m2), sy = fma(mxz, mxy, m2), sz = mxy * (mxz + myz);
-        if (ENERGY) {
-          ex = fma(sx, e, ex);
-          ey = fma(sy, e, ey);
-          rE = fma(sz, e, rE);
-        }
-        if (GRAD) {
-          const double W6 = fma(gx, sx, fma(gy, sy, rz.w * sz));
-          const double We = W6 * e;
-          const double H = fma(g, fma(alp * T, f, -1.5), -G2 * sabc);
-          // ds/da, ds/db, ds/dc from A = Y W3, B = X W3, C = X Y
-          const double A = Y * W3, Bq = X * W3, C = X * Y;
-          const double Ssum = A + Bq + C;
-          const double KW = K * W6;  // half of dL6/de-weighted factor; x2 at the end
-          const double al = KW * H, be = KW * G2;
-          const double ali = al * a * iabc;  // al / (b c)
-          const double ca = fma(al, ia, be * fma(-2.0, C, Ssum));
-          const double cb = fma(ali, cc, be * fma(-2.0, Bq, Ssum));
-          const double cg = fma(ali, bb, be * fma(-2.0, A, Ssum));
-          fx0 = fma(ca, dxy0, fma(cb, u0, fx0));
-          fx1 = fma(ca, dxy1, fma(cb, u1, fx1));
-          fx2 = fma(ca, dxy2, fma(cb, u2, fx2));
-          fy0 = fma(cg, v0, fma(-ca, dxy0, fy0));
-          fy1 = fma(cg, v1, fma(-ca, dxy1, fy1));
-          fy2 = fma(cg, v2, fma(-ca, dxy2, fy2));
-          rF0 = fma(-cb, u0, fma(-cg, v0, rF0));
-          rF1 = fma(-cb, u1, fma(-cg, v1, rF1));
-          rF2 = fma(-cb, u2, fma(-cg, v2, rF2));
-          cx = fma(We, kxy + zx.kc, cx);
-          cy = fma(We, kyx + zy.kc, cy);
-          rD = fma(We, zx.kz + zy.kz, rD);
-          if (PG) {
-            const double TdT = -6.0 * f * We * T;  // dL6/dT times T
-            g_s9 = fma(W6, P0 * f * g, g_s9);
-            g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
-            const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
-            const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
-            g_a1 = fma(TdT * alp / 3.0, Axy + sxz * bxz + syz * byz, g_a1);
-            g_a2 = fma(TdT * alp / 3.0, Bxy + bxz + byz, g_a2);
+      // FULL: every pair within the cutoff (weights sx = sy = sz = 2, the
+      // factor 2 applied when the sums are stored); FIRST: the chunk of y's
+      // own block, where third atoms up to y must be masked
+      auto steps = [&](auto full_c, auto first_c) {
+        constexpr bool FULL = decltype(full_c)::value, FIRST = decltype(first_c)::value;
+        D4_UNROLL(D4_ATM1_UNROLL)
+        for (int k = 0; k < kTile; ++k) {
+          const int zl = (k + tx) & (kTile - 1);
+          const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
+          const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
+          const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
+          const double2* px = reinterpret_cast<const double2*>(s_zc + zl * kZcRow + 4 * tx);
+          const double2* py =
+              reinterpret_cast<const double2*>(s_zc + zl * kZcRow + 4 * (kTile + ty));
+          const double2 x01 = px[0], y01 = py[0];
+          const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
+          const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
+          const double u0 = rx0 - rz.x, u1 = rx1 - rz.y, u2 = rx2 - rz.z;
+          const double v0 = ry0 - rz.x, v1 = ry1 - rz.y, v2 = ry2 - rz.z;
+          double bb = fma(u2, u2, fma(u1, u1, u0 * u0));  // r_xz^2
+          double cc = fma(v2, v2, fma(v1, v1, v0 * v0));  // r_yz^2
+          double P = pxy * x01.y * y01.y;                  // s9 is in pxy
+          double P0 = PG ? pxy0 * x01.y * y01.y : 0.0;     // without s9
+          if (FIRST) {
+            // third atoms up to y carry no weight (z may coincide with x or y)
+            const bool on = z0 + zl > y;
+            bb = on ? bb : 1.0;
+            cc = on ? cc : 1.0;
+            P = on ? P : 0.0;
+            if (PG) P0 = on ? P0 : 0.0;
+          }
+          const double T = txy * x01.x * y01.x;
+          const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
+          const double sabc = X * Y * W3;
+          const double bc = bb * cc;
+          const double abc = a * bc;
+          const double opT = fma(6.0, T, 1.0);
+          const double q0 = fast_rcp(abc * opT);
+          const double f = abc * q0;     // 1 / (1 + 6 T)
+          const double iabc = opT * q0;  // 1 / (a b c)
+          const double G2 = 0.375 * iabc;
+          const double G2s = G2 * sabc;
+          const double g = 1.0 + G2s;
+          const double K = P * f;
+          const double e = K * g;
+          double sx, sy, sz;
+          if (FULL) {
+            sx = sy = sz = 2.0;
+          } else {
+            const double mxz = bb <= c3sq ? 1.0 : 0.0, myz = cc <= c3sq ? 1.0 : 0.0;
+            const double m2 = mxz * myz;
+            sx = fma(myz, mxy, m2);
+            sy = fma(mxz, mxy, m2);
+            sz = mxy * (mxz + myz);
+          }
+          if (ENERGY) {
+            if (FULL) {
+              ex += e;  // x and y alike
+              rE += e;
+            } else {
+              ex = fma(sx, e, ex);
+              ey = fma(sy, e, ey);
+              rE = fma(sz, e, rE);
+            }
+          }
+          if (GRAD) {
+            const double W6 = FULL ? 2.0 * (gxy + rz.w) : fma(gx, sx, fma(gy, sy, rz.w * sz));
+            const double We = W6 * e;
+            const double H = fma(g, fma(alp * T, f, -1.5), -G2s);
+            // ds/da, ds/db, ds/dc from A = Y W3, B = X W3, C = X Y
+            const double A = Y * W3, Bq = X * W3, C = X * Y;
+            const double Ssum = A + Bq + C;
+            const double KW = K * W6;  // half of the dL/de-weighted factor; x2 at the end
+            const double al = KW * H, be = KW * G2;
+            const double ali = al * a * iabc;  // al / (b c)
+            const double ca = fma(al, ia, be * fma(-2.0, C, Ssum));
+            const double cb = fma(ali, cc, be * fma(-2.0, Bq, Ssum));
+            const double cg = fma(ali, bb, be * fma(-2.0, A, Ssum));
+            // x gets ca (x - y) + cb (x - z), y gets cg (y - z) - ca (x - y):
+            // the (x - y) part is summed as a scalar
+            sca += ca;
+            fx0 = fma(cb, u0, fx0);
+            fx1 = fma(cb, u1, fx1);
+            fx2 = fma(cb, u2, fx2);
+            fy0 = fma(cg, v0, fy0);
+            fy1 = fma(cg, v1, fy1);
+            fy2 = fma(cg, v2, fy2);
+            rF0 = fma(-cb, u0, fma(-cg, v0, rF0));
+            rF1 = fma(-cb, u1, fma(-cg, v1, rF1));
+            rF2 = fma(-cb, u2, fma(-cg, v2, rF2));
+            cx = fma(We, kxy + x23.y, cx);
+            cy = fma(We, kyx + y23.y, cy);
+            rD = fma(We, x23.x + y23.x, rD);
+            if (PG) {
+              const double TdT = -6.0 * f * We * T;  // dL6/dT times T
+              g_s9 = fma(W6, P0 * f * g, g_s9);
+              g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
+              const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
+              const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
+              g_a1 = fma(TdT * alp / 3.0, Axy + sxz * bxz + syz * byz, g_a1);
+              g_a2 = fma(TdT * alp / 3.0, Bxy + bxz + byz, g_a2);
+            }
+          }
+          if (ENERGY) rE = row_rotate_down(rE);
+          if (GRAD) {
+            rF0 = row_rotate_down(rF0);
+            rF1 = row_rotate_down(rF1);
+            rF2 = row_rotate_down(rF2);
+            rD = row_rotate_down(rD);
           }
         }
-        if (ENERGY) rE = row_rotate_down(rE);
-        if (GRAD) {
-          rF0 = row_rotate_down(rF0);
-          rF1 = row_rotate_down(rF1);
-          rF2 = row_rotate_down(rF2);
-          rD = row_rotate_down(rD);
-        }
+      };
+      const bool first = z0 == by * kTile;
+      if (full) {
+        if (first)
+          steps(std::true_type{}, std::true_type{});
+        else
+          steps(std::true_type{}, std::false_type{});
+      } else {
+        if (first)
+          steps(std::false_type{}, std::true_type{});
+        else
+          steps(std::false_type{}, std::false_type{});
       }
+      if (full) rE *= 2.0;
       // lane tx of every row now holds that row's sums for z0 + tx (the
       // gradient parts were accumulated at half weight)
       s_red[ty][tx][0] = rE;
@@ -1110,6 +1169,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
       }
     }
     // tile sums of the x atoms (over the rows) and the y atoms (over the columns)
+    if (full) {
+      ex *= 2.0;
+      ey = ex;
+    }
+    fx0 = fma(sca, dxy0, fx0);
+    fx1 = fma(sca, dxy1, fx1);
+    fx2 = fma(sca, dxy2, fx2);
+    fy0 = fma(-sca, dxy0, fy0);
+    fy1 = fma(-sca, dxy1, fy1);
+    fy2 = fma(-sca, dxy2, fy2);
     __syncthreads();
     s_red[ty][tx][0] = ex;
     s_red[ty][tx][1] = 2.0 * fx0;
