@@ -278,9 +278,10 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": ("k_atm1 (three-body, each triple once" if N <= 256 else
-                       "k_atm (three-body, pair tiles") + (", energy + gradient)" if forces
-                                                           else ", energy)"),
+            "kernel": ("k_atm (three-body, pair tiles, energy + gradient)" if N > 256 else
+                       "k_atm8 (three-body, each triple once, 8x8 pair tiles, energy + gradient)"
+                       if forces else
+                       "k_atm1 (three-body, each triple once, 16x16 pair tiles, energy)"),
             "bound": "mfma",
             "note": "FP64 VALU kernel (no MFMA); MI355X FP64 peak is 78.6 TFLOP/s for vector "
                     "and matrix alike",
