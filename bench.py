@@ -278,7 +278,8 @@ def main() -> int:
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": ("k_atm (three-body, pair tiles, energy + gradient)" if N > 256 else
+            "kernel": ("k_atm8s (three-body, each triple once, 8x8 pair tiles, group screening, "
+                       "energy + gradient)" if N > 256 else
                        "k_atm8 (three-body, each triple once, 8x8 pair tiles, energy + gradient)"
                        if forces else
                        "k_atm1 (three-body, each triple once, 16x16 pair tiles, energy)"),

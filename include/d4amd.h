@@ -40,10 +40,12 @@ extern "C" {
 #define D4_FLAG_FORCE_LU 2    /* testing: solve every EEQ system by pivoted LU */
 #define D4_FLAG_FORCE_TILED 4 /* testing: use the large-system (tiled) EEQ path at any size */
 #define D4_FLAG_FORCE_SORT 8  /* testing: spatially sort the atoms of every molecule */
-#define D4_FLAG_FORCE_PAIR_TILES 16 /* testing: three-body term by pair tiles at any size */
+#define D4_FLAG_FORCE_PAIR_TILES 16 /* testing: the large-system three-body kernel at any size */
 #define D4_FLAG_D4S 32     /* D4S model: Gaussian weights with pairwise factors
                               (reference src/tad_dftd4/model/d4s.py:56-285) */
 #define D4_FLAG_ATM16 64   /* testing: one-pass three-body kernel with 16 x 16 pair tiles */
+#define D4_FLAG_PAIR_OWNER 128 /* testing: large systems take the pair-owner three-body
+                                  kernel (each triple from its three pairs) */
 
 /* Element and reference-system tables, host memory, read once by
  * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference

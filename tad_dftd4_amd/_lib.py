@@ -28,6 +28,7 @@ D4_FLAG_FORCE_SORT = 8
 D4_FLAG_FORCE_PAIR_TILES = 16
 D4_FLAG_D4S = 32
 D4_FLAG_ATM16 = 64
+D4_FLAG_PAIR_OWNER = 128
 
 SLOTS = ("prep", "cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain")
 

@@ -83,6 +83,7 @@ d4::Params to_params(const d4_params* p) {
   k.force_pairtiles = (p->flags & D4_FLAG_FORCE_PAIR_TILES) ? 1 : 0;
   k.d4s = (p->flags & D4_FLAG_D4S) ? 1 : 0;
   k.atm16 = (p->flags & D4_FLAG_ATM16) ? 1 : 0;
+  k.pair_owner = (p->flags & D4_FLAG_PAIR_OWNER) ? 1 : 0;
   return k;
 }
 

@@ -53,6 +53,7 @@ struct Params {
   int pgrad;  // accumulate parameter gradients (backward only)
   int d4s;    // D4S model: Gaussian weights with pairwise factors
   int atm16;  // one-pass three-body kernel with 16 x 16 pair tiles (testing)
+  int pair_owner;  // large systems: pair-owner three-body kernel (testing)
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).
@@ -81,7 +82,8 @@ struct Work {
   double* pt;       // [B*N*N] ATM damping factor t_ij = (R0_ij / r_ij)^(alp/3)
   double* pp;       // [B*N*N] ATM factor p_ij = sqrt|C6(q=0)_ij| / r_ij^3
   double* pk;       // [B*N*N] d log sqrt|C6(q=0)_ij| / d CN_i
-  double* pg;       // [B*N*N] ATM dL/d r_ij^2
+  double* pg;       // [B*N*N] ATM dL/d r_ij^2; pg, ph and pe together hold the window
+                    // buffer of the screened one-pass kernel (atm8s_stride per molecule)
   double* ph;       // [B*N*N] ATM dL/d log sqrt|C6(q=0)_ij|
   double* pe;       // [B*N*N] ATM atom-energy partials (row atom)
   // spatial order of large molecules (see launch_sort)
@@ -94,6 +96,8 @@ struct Work {
   double* sE;       // [B*N] per-atom outputs in sorted order
   double* sG;       // [B*N*3] gradient in sorted order
   double* gsph;     // [B*NG*4] bounding sphere (centre, radius) of each 16-atom group
+  double* gsph8;    // [B*NG8*4] bounding sphere of each 8-atom group
+  int64_t* wix;     // [B*(2*NG8+3)] slot tables of the screened one-pass kernel
   double* apart;    // [B*S*5*N] per-split atom sums of the one-pass three-body kernel
   double* rt4;      // [B*N] sqrt(r4r2) per atom (parameter gradients)
   double* pg2;      // [B*N*5] two-body parameter-gradient partials per atom row
@@ -124,15 +128,63 @@ __host__ __device__ inline int atm8_splits(int n) {
   s = s > kAtm8MaxSplits ? kAtm8MaxSplits : s;
   return tiles < 1 ? 1 : (s < 1 ? 1 : (s > tiles ? tiles : s));
 }
+
+// Screened one-pass three-body kernel (large systems, atoms in spatial
+// order): 8-atom groups; x-group gx of a molecule with G groups is split over
+// atm8s_splits(G, gx, smax) work items, about smax ((G - gx) / G)^2 (the
+// work of gx falls with its forward neighbourhood), smax = atm8s_smax(n) at
+// most kAtm8sMaxSplits. Item (gx, s) has one window slot (5 x 8 doubles) for
+// each group g >= gx; the slots of a molecule fill at most atm8s_budget(n)
+// doubles (12 n^2 from kAtm1MaxN to 16384 atoms, else 3 n^2: the pair-matrix
+// space pg, ph, pe is allocated atm8s_stride(N) doubles per molecule), and
+// smax is the largest power of two that fits. All of it depends on the atom
+// count n alone.
+constexpr int kAtm8sMaxSplits = 64;
+__host__ __device__ inline int64_t n_groups8(int64_t N) { return (N + kAtm8 - 1) / kAtm8; }
+__host__ __device__ inline int atm8s_splits(int G, int gx, int smax) {
+  const long long r = G - gx, GG = (long long)G * G;
+  const long long s = (smax * r * r + GG - 1) / GG;
+  return s < 1 ? 1 : (s > smax ? smax : (int)s);
+}
+__host__ __device__ inline long long atm8s_slots(int G, int smax) {
+  long long t = 0;
+  for (int gx = 0; gx < G; ++gx) t += (long long)atm8s_splits(G, gx, smax) * (G - gx);
+  return t;
+}
+constexpr int64_t kAtm8sWideMax = 16384;
+__host__ __device__ inline int64_t atm8s_budget(int64_t n) {
+  const int64_t a = (n > kAtm1MaxN && n <= kAtm8sWideMax ? 12 : 3) * n * n;
+  return a > 48 ? a : 48;
+}
+// the largest budget of a molecule of up to N atoms
+__host__ __device__ inline int64_t atm8s_stride(int64_t N) {
+  const int64_t a = atm8s_budget(N), b = atm8s_budget(N < kAtm8sWideMax ? N : kAtm8sWideMax);
+  return a > b ? a : b;
+}
+__host__ __device__ inline int atm8s_smax(int n) {
+  const int G = (n + kAtm8 - 1) / kAtm8;
+  for (int s = kAtm8sMaxSplits; s > 1; s >>= 1)
+    if (atm8s_slots(G, s) * kAtm1Acc * kAtm8 <= atm8s_budget(n)) return s;
+  return 1;
+}
+// Work-groups per molecule (x-groups padded to a multiple of 8, times the
+// largest split count) and the length of a molecule's slot table.
+__host__ __device__ inline int64_t atm8s_blocks_per_mol(int64_t N) {
+  return (n_groups8(N) + 7) / 8 * 8 * kAtm8sMaxSplits;
+}
+__host__ __device__ inline int64_t atm8s_table_len(int64_t N) { return 2 * n_groups8(N) + 3; }
+
 // Three-body parameter-gradient partials per molecule: one per split of the
-// one-pass kernel or one per pair tile of the pair-tile kernel (the larger).
+// one-pass kernels or one per pair tile of the pair-tile kernel (the largest).
 __host__ __device__ inline int64_t pgrad_partials(int64_t N) {
   const int64_t nb = n_groups(N);
   const int64_t tiles = nb * (nb + 1) / 2;
   int64_t s = N <= kAtm1MaxN ? atm1_splits((int)N) : 0;
   const int64_t s8 = N <= kAtm1MaxN ? atm8_splits((int)N) : 0;
   s = s > s8 ? s : s8;
-  return tiles > s ? tiles : s;
+  s = s > tiles ? s : tiles;
+  const int64_t s8s = atm8s_blocks_per_mol(N);
+  return s > s8s ? s : s8s;
 }
 __host__ __device__ inline int atm1_splits(int n) {
   const int nb = (n + kGroup - 1) / kGroup;
@@ -201,7 +253,12 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
   v.pt = (double*)take(8 * bn * size_t(N));
   v.pp = (double*)take(8 * bn * size_t(N));
   v.pk = (double*)take(8 * bn * size_t(N));
-  v.pg = (double*)take(8 * bn * size_t(N));
+  {
+    // pg, ph and pe also hold the window buffer of the screened one-pass
+    // kernel: atm8s_stride(N) doubles per molecule from pg on
+    const size_t nn = size_t(N) * size_t(N), st = size_t(atm8s_stride(N));
+    v.pg = (double*)take(8 * size_t(B) * (st > 3 * nn ? st - 2 * nn : nn));
+  }
   v.ph = (double*)take(8 * bn * size_t(N));
   v.pe = (double*)take(8 * bn * size_t(N));
   v.perm = (int32_t*)take(sizeof(int32_t) * bn);
@@ -213,6 +270,8 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
   v.sE = (double*)take(8 * bn);
   v.sG = (double*)take(24 * bn);
   v.gsph = (double*)take(32 * size_t(B) * size_t(n_groups(N)));
+  v.gsph8 = (double*)take(32 * size_t(B) * size_t(n_groups8(N)));
+  v.wix = (int64_t*)take(8 * size_t(B) * size_t(atm8s_table_len(N)));
   v.apart = (double*)take(N <= kAtm1MaxN
                               ? 8 * size_t(B) * size_t(one_pass_split_max((int)N)) * kAtm1Acc *
                                     size_t(N)
@@ -279,5 +338,6 @@ hipError_t launch_scatter_atoms(hipStream_t s, const Work& w, const double* src,
 hipError_t launch_scatter_pairs(hipStream_t s, const Work& w, const double* src,
                                 double* dst);
 hipError_t launch_groups(hipStream_t s, const Inputs& in, const Work& w);
+hipError_t launch_groups8(hipStream_t s, const Inputs& in, const Work& w);
 
 }  // namespace d4

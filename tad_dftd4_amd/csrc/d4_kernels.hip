@@ -1257,15 +1257,313 @@ __device__ __forceinline__ double xor_lanes8(double v) {
   return dpp_double<D == 1 ? kDppXor1 : (D == 3 ? kDppXor3 : kDppHalfMirror)>(v);
 }
 
+// Running sums for the third atoms of a chunk, handed around the 8-lane ring.
+struct Atm8Ring {
+  double E, D, F0, F1, F2;  // energy share, dL/dCN, gradient (half weight)
+};
+
 template <int D>
-__device__ __forceinline__ void ring_xor8(double& a, double& b, double& c, double& d, double& e,
-                                          bool energy, bool grad) {
-  if (energy) a = xor_lanes8<D>(a);
+__device__ __forceinline__ void ring_xor8(Atm8Ring& r, bool energy, bool grad) {
+  if (energy) r.E = xor_lanes8<D>(r.E);
   if (grad) {
-    b = xor_lanes8<D>(b);
-    c = xor_lanes8<D>(c);
-    d = xor_lanes8<D>(d);
-    e = xor_lanes8<D>(e);
+    r.D = xor_lanes8<D>(r.D);
+    r.F0 = xor_lanes8<D>(r.F0);
+    r.F1 = xor_lanes8<D>(r.F1);
+    r.F2 = xor_lanes8<D>(r.F2);
+  }
+}
+
+constexpr double kAtmFar = 1.0e6;  // position of lanes and third atoms without an atom (bohr)
+// [z][x block | y block] pair records (t, p, kz, kc) of a chunk; the row
+// stride of 33 16-byte units (odd) keeps the skewed reads of a row on
+// distinct banks
+constexpr int kAtm8Row = 66;
+
+// The pair (x, y) a lane owns. Lanes without a pair sit far from every atom
+// (no zero distance) with zero pair factors.
+struct Atm8Pair {
+  double rx0, rx1, rx2, ry0, ry1, ry2;  // positions
+  double a, ia;                         // r_xy^2 and 1 / r_xy^2
+  double txy, pxy, pxy0;                // t_xy, s9 p_xy, p_xy
+  double kxy, kyx;                      // d log sqrt|C6(0)_xy| / d CN_x, / d CN_y
+  double gx, gy, gxy;                   // dL/dE of x, y and their sum
+  double mxy;                           // 1 if r_xy is within the cutoff
+  double rx4, ry4, Axy, Bxy;            // parameter gradients
+};
+
+template <bool GRAD, bool PG>
+__device__ __forceinline__ Atm8Pair atm8_pair(const Params& p, const Inputs& in, const Work& w,
+                                              int64_t m, int x, int y, bool valid) {
+  const int64_t N = w.N;
+  const double* R = in.positions + m * N * 3;
+  Atm8Pair q;
+  q.rx0 = q.rx1 = q.rx2 = q.ry0 = q.ry1 = q.ry2 = -kAtmFar;
+  q.a = q.ia = 1.0;
+  q.txy = q.pxy = q.pxy0 = q.kxy = q.kyx = 0.0;
+  q.gx = q.gy = q.mxy = 0.0;
+  q.rx4 = q.ry4 = q.Axy = q.Bxy = 0.0;
+  if (valid) {
+    const double* pt = w.pt + m * N * N;
+    const double* pp = w.pp + m * N * N;
+    const double* pk = w.pk + m * N * N;
+    q.rx0 = R[3 * x];
+    q.rx1 = R[3 * x + 1];
+    q.rx2 = R[3 * x + 2];
+    q.ry0 = R[3 * y];
+    q.ry1 = R[3 * y + 1];
+    q.ry2 = R[3 * y + 2];
+    const double d0 = q.rx0 - q.ry0, d1 = q.rx1 - q.ry1, d2 = q.rx2 - q.ry2;
+    q.a = d0 * d0 + d1 * d1 + d2 * d2;
+    q.ia = 1.0 / q.a;
+    q.txy = pt[(int64_t)x * N + y];
+    q.pxy0 = pp[(int64_t)x * N + y];
+    q.pxy = p.s9 * q.pxy0;  // s9 folded into the pair factor
+    q.mxy = q.a <= p.cut3sq ? 1.0 : 0.0;
+    if (GRAD) {
+      q.kxy = pk[(int64_t)x * N + y];
+      q.kyx = pk[(int64_t)y * N + x];
+      q.gx = in.gE ? in.gE[m * N + x] : 1.0;
+      q.gy = in.gE ? in.gE[m * N + y] : 1.0;
+    }
+    if (PG) {
+      q.rx4 = w.rt4[m * N + x];
+      q.ry4 = w.rt4[m * N + y];
+      const double sig = 1.7320508075688772 * q.rx4 * q.ry4;
+      q.Bxy = 1.0 / fma(p.a1, sig, p.a2);
+      q.Axy = sig * q.Bxy;
+    }
+  }
+  q.gxy = q.gx + q.gy;
+  return q;
+}
+
+// Sums a lane keeps over its chunks: shares of x and y, their gradient parts
+// (the (x - y) part as the scalar sca) and the parameter gradients (x 6).
+struct Atm8Sums {
+  double ex, ey, cx, cy, sca, fx0, fx1, fx2, fy0, fy1, fy2;
+  double g_s9, g_alp, g_a1, g_a2;
+};
+
+// Records of one chunk (8 third atoms x 16 columns, two per lane) and the
+// third-atom data (lanes 0..7), held in registers until staged.
+struct Atm8Chunk {
+  double t[2], p[2], kz[2], kc[2], r[5];
+};
+
+// Load the chunk of third atoms z0 .. z0 + 7 against the x block starting at
+// atom x0 and the y block starting at y0.
+template <bool GRAD, bool PG>
+__device__ __forceinline__ void atm8_fetch(Atm8Chunk& c, const Inputs& in, const Work& w,
+                                           int64_t m, int n, int z0, int x0, int y0, int lane) {
+  constexpr int T8 = kAtm8;
+  const int64_t N = w.N;
+  const double* pt = w.pt + m * N * N;
+  const double* pp = w.pp + m * N * N;
+  const double* pk = w.pk + m * N * N;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int e = lane + 64 * r;
+    const int zl = e >> 4, cl = e & 15;
+    const int z = z0 + zl;
+    const int col = cl < T8 ? x0 + cl : y0 + cl - T8;
+    c.t[r] = c.p[r] = c.kz[r] = c.kc[r] = 0.0;
+    if (z < n && col < n) {
+      const int64_t idx = (int64_t)z * N + col;
+      c.t[r] = pt[idx];
+      c.p[r] = pp[idx];
+      if (GRAD) {
+        c.kz[r] = pk[idx];
+        c.kc[r] = pk[(int64_t)col * N + z];
+      }
+    }
+  }
+  if (lane < T8) {
+    const int z = z0 + lane;
+    const bool ok = z < n && in.numbers[m * N + z] != 0;
+    const double* R = in.positions + m * N * 3;
+    c.r[0] = ok ? R[3 * z] : kAtmFar;
+    c.r[1] = ok ? R[3 * z + 1] : kAtmFar;
+    c.r[2] = ok ? R[3 * z + 2] : kAtmFar;
+    c.r[3] = (GRAD && ok) ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0;
+    c.r[4] = (PG && ok) ? w.rt4[m * N + z] : 0.0;
+  }
+}
+
+template <bool PG>
+__device__ __forceinline__ void atm8_stage(const Atm8Chunk& c, double* s_zc, double* s_r,
+                                           double* s_rt, int lane) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int e = lane + 64 * r;
+    const int zl = e >> 4, cl = e & 15;
+    double2* dst = reinterpret_cast<double2*>(s_zc + zl * kAtm8Row + 4 * cl);
+    dst[0] = make_double2(c.t[r], c.p[r]);
+    dst[1] = make_double2(c.kz[r], c.kc[r]);
+  }
+  if (lane < kAtm8) {
+    double* r = s_r + 6 * lane;
+    r[0] = c.r[0];
+    r[1] = c.r[1];
+    r[2] = c.r[2];
+    r[3] = c.r[3];
+    if (PG) s_rt[lane] = c.r[4];
+  }
+}
+
+// The 8 steps of one staged chunk (third atoms z0 .. z0 + 7) for the pair of
+// lane (tx, ty). FULL: every pair within the cutoff (weights sx = sy = sz =
+// 2; the factor 2 of the energies is applied by the caller); FIRST: the chunk
+// of y's own block, where third atoms up to y carry no weight. On return lane
+// tx of every row holds that row's ring sums for z0 + (tx ^ 7).
+template <bool ENERGY, bool GRAD, bool PG, bool FULL, bool FIRST>
+__device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, Atm8Sums& u,
+                                           Atm8Ring& rr, const double* s_zc, const double* s_r,
+                                           const double* s_rt, int tx, int ty, int z0, int y) {
+  constexpr int T8 = kAtm8;
+  const double c3sq = p.cut3sq, alp = p.alp, a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
+#pragma unroll 2
+  for (int k = 0; k < T8; ++k) {
+    const int zl = tx ^ k;
+    const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
+    const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
+    const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
+    const double* zrow = s_zc + __umul24(zl, kAtm8Row);
+    const double2* px = reinterpret_cast<const double2*>(zrow + 4 * tx);
+    const double2* py = reinterpret_cast<const double2*>(zrow + 4 * (T8 + ty));
+    const double2 x01 = px[0], y01 = py[0];
+    const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
+    const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
+    const double u0 = q.rx0 - rz.x, u1 = q.rx1 - rz.y, u2 = q.rx2 - rz.z;
+    const double v0 = q.ry0 - rz.x, v1 = q.ry1 - rz.y, v2 = q.ry2 - rz.z;
+    double bb = fma(u2, u2, fma(u1, u1, u0 * u0));  // r_xz^2
+    double cc = fma(v2, v2, fma(v1, v1, v0 * v0));  // r_yz^2
+    double P = q.pxy * x01.y * y01.y;                // s9 is in pxy
+    double P0 = PG ? q.pxy0 * x01.y * y01.y : 0.0;   // without s9
+    if (FIRST) {
+      // third atoms up to y carry no weight (z may coincide with x or y)
+      const bool on = z0 + zl > y;
+      bb = on ? bb : 1.0;
+      cc = on ? cc : 1.0;
+      P = on ? P : 0.0;
+      if (PG) P0 = on ? P0 : 0.0;
+    }
+    const double a = q.a;
+    const double T = q.txy * x01.x * y01.x;
+    const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
+    const double sabc = X * Y * W3;
+    const double bc = bb * cc;
+    const double abc = a * bc;
+    const double opT = fma(6.0, T, 1.0);
+    const double q0 = fast_rcp(abc * opT);
+    const double f = abc * q0;     // 1 / (1 + 6 T)
+    const double iabc = opT * q0;  // 1 / (a b c)
+    const double G2 = 0.375 * iabc;
+    const double G2s = G2 * sabc;
+    const double g = 1.0 + G2s;
+    const double K = P * f;
+    const double e = K * g;
+    double sx, sy, sz;
+    if (FULL) {
+      sx = sy = sz = 2.0;
+    } else {
+      const double mxz = bb <= c3sq ? 1.0 : 0.0, myz = cc <= c3sq ? 1.0 : 0.0;
+      const double m2 = mxz * myz;
+      sx = fma(myz, q.mxy, m2);
+      sy = fma(mxz, q.mxy, m2);
+      sz = q.mxy * (mxz + myz);
+    }
+    if (ENERGY) {
+      if (FULL) {
+        u.ex += e;  // x and y alike
+        rr.E += e;
+      } else {
+        u.ex = fma(sx, e, u.ex);
+        u.ey = fma(sy, e, u.ey);
+        rr.E = fma(sz, e, rr.E);
+      }
+    }
+    if (GRAD) {
+      const double W6 = FULL ? 2.0 * (q.gxy + rz.w) : fma(q.gx, sx, fma(q.gy, sy, rz.w * sz));
+      const double We = W6 * e;
+      const double H = fma(g, fma(alp * T, f, -1.5), -G2s);
+      // ds/da, ds/db, ds/dc from A = Y W3, B = X W3, C = X Y
+      const double A = Y * W3, Bq = X * W3, C = X * Y;
+      const double Ssum = A + Bq + C;
+      const double KW = K * W6;  // half of the dL/de-weighted factor; x2 at the end
+      const double al = KW * H, be = KW * G2;
+      const double ali = al * a * iabc;  // al / (b c)
+      const double ca = fma(al, q.ia, be * fma(-2.0, C, Ssum));
+      const double cb = fma(ali, cc, be * fma(-2.0, Bq, Ssum));
+      const double cg = fma(ali, bb, be * fma(-2.0, A, Ssum));
+      // x gets ca (x - y) + cb (x - z), y gets cg (y - z) - ca (x - y):
+      // the (x - y) part is summed as a scalar
+      u.sca += ca;
+      u.fx0 = fma(cb, u0, u.fx0);
+      u.fx1 = fma(cb, u1, u.fx1);
+      u.fx2 = fma(cb, u2, u.fx2);
+      u.fy0 = fma(cg, v0, u.fy0);
+      u.fy1 = fma(cg, v1, u.fy1);
+      u.fy2 = fma(cg, v2, u.fy2);
+      rr.F0 = fma(-cb, u0, fma(-cg, v0, rr.F0));
+      rr.F1 = fma(-cb, u1, fma(-cg, v1, rr.F1));
+      rr.F2 = fma(-cb, u2, fma(-cg, v2, rr.F2));
+      u.cx = fma(We, q.kxy + x23.y, u.cx);
+      u.cy = fma(We, q.kyx + y23.y, u.cy);
+      rr.D = fma(We, x23.x + y23.x, rr.D);
+      if (PG) {
+        const double TdT = -6.0 * f * We * T;  // dL6/dT times T
+        u.g_s9 = fma(W6, P0 * f * g, u.g_s9);
+        u.g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, u.g_alp);
+        const double sxz = sq3 * q.rx4 * s_rt[zl], syz = sq3 * q.ry4 * s_rt[zl];
+        const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
+        u.g_a1 = fma(TdT * alp / 3.0, q.Axy + sxz * bxz + syz * byz, u.g_a1);
+        u.g_a2 = fma(TdT * alp / 3.0, q.Bxy + bxz + byz, u.g_a2);
+      }
+    }
+    // hand the sums for z to the lane that takes z at the next step:
+    // k ^ (k + 1) is 1, 3, 1, 7, 1, 3, 1 (only the last step of each group
+    // of four depends on the group)
+    if ((k & 1) == 0)
+      ring_xor8<1>(rr, ENERGY, GRAD);
+    else if ((k & 3) == 1)
+      ring_xor8<3>(rr, ENERGY, GRAD);
+    else if (k == 3)
+      ring_xor8<7>(rr, ENERGY, GRAD);
+  }
+}
+
+// Fold the (x - y) force scalar of a tile into the x and y gradient sums.
+__device__ __forceinline__ void atm8_fold_sca(const Atm8Pair& q, Atm8Sums& u) {
+  const double d0 = q.rx0 - q.ry0, d1 = q.rx1 - q.ry1, d2 = q.rx2 - q.ry2;
+  u.fx0 = fma(u.sca, d0, u.fx0);
+  u.fx1 = fma(u.sca, d1, u.fx1);
+  u.fx2 = fma(u.sca, d2, u.fx2);
+  u.fy0 = fma(-u.sca, d0, u.fy0);
+  u.fy1 = fma(-u.sca, d1, u.fy1);
+  u.fy2 = fma(-u.sca, d2, u.fy2);
+  u.sca = 0.0;
+}
+
+// Stage five per-lane values (energy share, gradient at half weight, dL/dCN)
+// in s_red for the sums over rows or columns of the tile.
+template <int NA>
+__device__ __forceinline__ void atm8_stage_sums(double (*s_red)[64], int lane, double e,
+                                                double f0, double f1, double f2, double c) {
+  s_red[0][lane] = e;
+  s_red[1][lane] = 2.0 * f0;
+  s_red[2][lane] = 2.0 * f1;
+  s_red[3][lane] = 2.0 * f2;
+  s_red[4][lane] = c;
+}
+
+// Parameter-gradient partials of a wave: s9, alp, a1, a2 (the sums carry a
+// factor 6 from the screening weights).
+__device__ __forceinline__ void atm8_store_pgrad(const Atm8Sums& u, int lane, double* out) {
+  const double sixth = 1.0 / 6.0;
+  const double v4[4] = {u.g_s9 * sixth, u.g_alp * sixth, u.g_a1 * sixth, u.g_a2 * sixth};
+  for (int k = 0; k < 4; ++k) {
+    const double v = wave_sum(v4[k]);
+    if (lane == 0) out[k] = v;
   }
 }
 
@@ -1274,11 +1572,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     Params p, Inputs in, Work w, int S, int64_t nblocks) {
   constexpr int NA = kAtm1Acc;
   constexpr int T8 = kAtm8;
-  constexpr double kFar = 1.0e6;  // position of padding third atoms (bohr)
-  // [z][x block | y block] pair records (t, p, kz, kc); the row stride of 33
-  // 16-byte units (odd) keeps the skewed reads of a row on distinct banks
-  constexpr int kRow = 66;
-  __shared__ __align__(16) double s_zc[T8 * kRow];
+  __shared__ __align__(16) double s_zc[T8 * kAtm8Row];
   __shared__ __align__(16) double s_r[T8 * 6];  // z position, dL/dE_z
   __shared__ double s_rt[T8];                   // sqrt(r4r2_z) (parameter gradients)
   __shared__ double s_red[NA][64];
@@ -1298,12 +1592,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const int lane = threadIdx.x, tx = lane & (T8 - 1), ty = lane >> 3;
   const int64_t* Z = in.numbers + m * N;
   const double* R = in.positions + m * N * 3;
-  const double* pt = w.pt + m * N * N;
-  const double* pp = w.pp + m * N * N;
-  const double* pk = w.pk + m * N * N;
-  const double c3sq = p.cut3sq, s9 = p.s9, alp = p.alp;
+  const double c3sq = p.cut3sq;
   for (int e = lane; e < NA * n; e += 64) s_acc[e] = 0.0;
-  double g_s9 = 0.0, g_alp = 0.0, g_a1 = 0.0, g_a2 = 0.0;  // parameter gradients (x 6)
+  Atm8Sums u = {};
   // molecules whose bounding box fits in the cutoff sphere have every pair
   // within the cutoff: all screening weights are 2 and their masks drop out
   bool full;
@@ -1322,7 +1613,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                  d2 = wave_max(hi2) + wave_max(-lo2);
     full = d0 * d0 + d1 * d1 + d2 * d2 <= c3sq;
   }
-  const double a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
 
   for (int t = s; t < tiles; t += Sm) {
     int by = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
@@ -1331,234 +1621,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int bx = t - by * (by + 1) / 2;
     const int x = bx * T8 + tx, y = by * T8 + ty;
     const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
-    // lanes without a pair sit far from every atom (no zero distance; their
-    // pair factor is zero)
-    double rx0 = -kFar, rx1 = -kFar, rx2 = -kFar, ry0 = -kFar, ry1 = -kFar, ry2 = -kFar;
-    double a = 1.0, ia = 1.0, txy = 0.0, pxy = 0.0, pxy0 = 0.0, kxy = 0.0, kyx = 0.0;
-    double gx = 0.0, gy = 0.0, mxy = 0.0;
-    if (valid) {
-      rx0 = R[3 * x];
-      rx1 = R[3 * x + 1];
-      rx2 = R[3 * x + 2];
-      ry0 = R[3 * y];
-      ry1 = R[3 * y + 1];
-      ry2 = R[3 * y + 2];
-      const double d0 = rx0 - ry0, d1 = rx1 - ry1, d2 = rx2 - ry2;
-      a = d0 * d0 + d1 * d1 + d2 * d2;
-      ia = 1.0 / a;
-      txy = pt[(int64_t)x * N + y];
-      pxy0 = pp[(int64_t)x * N + y];
-      pxy = s9 * pxy0;  // s9 folded into the pair factor
-      mxy = a <= c3sq ? 1.0 : 0.0;
-      if (GRAD) {
-        kxy = pk[(int64_t)x * N + y];
-        kyx = pk[(int64_t)y * N + x];
-        gx = in.gE ? in.gE[m * N + x] : 1.0;
-        gy = in.gE ? in.gE[m * N + y] : 1.0;
-      }
-    }
-    const double dxy0 = rx0 - ry0, dxy1 = rx1 - ry1, dxy2 = rx2 - ry2;
-    double rx4 = 0.0, ry4 = 0.0, Axy = 0.0, Bxy = 0.0;
-    if (PG && valid) {
-      rx4 = w.rt4[m * N + x];
-      ry4 = w.rt4[m * N + y];
-      const double sig = sq3 * rx4 * ry4;
-      Bxy = 1.0 / fma(a1p, sig, a2p);
-      Axy = sig * Bxy;
-    }
-    const double gxy = gx + gy;
-    double ex = 0.0, ey = 0.0, cx = 0.0, cy = 0.0, sca = 0.0;
-    double fx0 = 0.0, fx1 = 0.0, fx2 = 0.0, fy0 = 0.0, fy1 = 0.0, fy2 = 0.0;
+    const Atm8Pair q = atm8_pair<GRAD, PG>(p, in, w, m, x, y, valid);
+    u.ex = u.ey = u.cx = u.cy = u.sca = 0.0;
+    u.fx0 = u.fx1 = u.fx2 = u.fy0 = u.fy1 = u.fy2 = 0.0;
 
-    // records of a chunk (8 z x 16 columns, two per lane) and the third-atom
-    // data (lanes 0..7), loaded one chunk ahead into registers
-    double vt[2], vp[2], vkz[2], vkc[2], vr[5];
-    auto fetch = [&](int z0) {
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const int e = lane + 64 * r;
-        const int zl = e >> 4, c = e & 15;
-        const int z = z0 + zl;
-        const int col = c < T8 ? bx * T8 + c : by * T8 + c - T8;
-        vt[r] = vp[r] = vkz[r] = vkc[r] = 0.0;
-        if (z < n && col < n) {
-          const int64_t idx = (int64_t)z * N + col;
-          vt[r] = pt[idx];
-          vp[r] = pp[idx];
-          if (GRAD) {
-            vkz[r] = pk[idx];
-            vkc[r] = pk[(int64_t)col * N + z];
-          }
-        }
-      }
-      if (lane < T8) {
-        const int z = z0 + lane;
-        const bool ok = z < n && Z[z] != 0;
-        vr[0] = ok ? R[3 * z] : kFar;
-        vr[1] = ok ? R[3 * z + 1] : kFar;
-        vr[2] = ok ? R[3 * z + 2] : kFar;
-        vr[3] = (GRAD && ok) ? (in.gE ? in.gE[m * N + z] : 1.0) : 0.0;
-        vr[4] = (PG && ok) ? w.rt4[m * N + z] : 0.0;
-      }
-    };
-    fetch(by * T8);
-
+    Atm8Chunk ch;
+    atm8_fetch<GRAD, PG>(ch, in, w, m, n, by * T8, bx * T8, by * T8, lane);
     for (int z0 = by * T8; z0 < n; z0 += T8) {
       __syncthreads();
-#pragma unroll
-      for (int r = 0; r < 2; ++r) {
-        const int e = lane + 64 * r;
-        const int zl = e >> 4, c = e & 15;
-        double2* dst = reinterpret_cast<double2*>(s_zc + zl * kRow + 4 * c);
-        dst[0] = make_double2(vt[r], vp[r]);
-        dst[1] = make_double2(vkz[r], vkc[r]);
-      }
-      if (lane < T8) {
-        double* r = s_r + 6 * lane;
-        r[0] = vr[0];
-        r[1] = vr[1];
-        r[2] = vr[2];
-        r[3] = vr[3];
-        if (PG) s_rt[lane] = vr[4];
-      }
+      atm8_stage<PG>(ch, s_zc, s_r, s_rt, lane);
       __syncthreads();
-      if (z0 + T8 < n) fetch(z0 + T8);  // in flight during this chunk
+      if (z0 + T8 < n)  // in flight during this chunk
+        atm8_fetch<GRAD, PG>(ch, in, w, m, n, z0 + T8, bx * T8, by * T8, lane);
 
-      double rE = 0.0, rD = 0.0, rF0 = 0.0, rF1 = 0.0, rF2 = 0.0;  // ring sums
-      // FULL: every pair within the cutoff (weights sx = sy = sz = 2, the
-      // factor 2 applied when the sums are stored); FIRST: the chunk of y's
-      // own block, where third atoms up to y must be masked
-      auto steps = [&](auto full_c, auto first_c) {
-        constexpr bool FULL = decltype(full_c)::value, FIRST = decltype(first_c)::value;
-#pragma unroll 2
-        for (int k = 0; k < T8; ++k) {
-          const int zl = tx ^ k;
-          const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
-          const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
-          const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
-          const double* zrow = s_zc + __umul24(zl, kRow);
-          const double2* px = reinterpret_cast<const double2*>(zrow + 4 * tx);
-          const double2* py = reinterpret_cast<const double2*>(zrow + 4 * (T8 + ty));
-          const double2 x01 = px[0], y01 = py[0];
-          const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
-          const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
-          const double u0 = rx0 - rz.x, u1 = rx1 - rz.y, u2 = rx2 - rz.z;
-          const double v0 = ry0 - rz.x, v1 = ry1 - rz.y, v2 = ry2 - rz.z;
-          double bb = fma(u2, u2, fma(u1, u1, u0 * u0));  // r_xz^2
-          double cc = fma(v2, v2, fma(v1, v1, v0 * v0));  // r_yz^2
-          double P = pxy * x01.y * y01.y;                  // s9 is in pxy
-          double P0 = PG ? pxy0 * x01.y * y01.y : 0.0;     // without s9
-          if (FIRST) {
-            // third atoms up to y carry no weight (z may coincide with x or y)
-            const bool on = z0 + zl > y;
-            bb = on ? bb : 1.0;
-            cc = on ? cc : 1.0;
-            P = on ? P : 0.0;
-            if (PG) P0 = on ? P0 : 0.0;
-          }
-          const double T = txy * x01.x * y01.x;
-          const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
-          const double sabc = X * Y * W3;
-          const double bc = bb * cc;
-          const double abc = a * bc;
-          const double opT = fma(6.0, T, 1.0);
-          const double q0 = fast_rcp(abc * opT);
-          const double f = abc * q0;     // 1 / (1 + 6 T)
-          const double iabc = opT * q0;  // 1 / (a b c)
-          const double G2 = 0.375 * iabc;
-          const double G2s = G2 * sabc;
-          const double g = 1.0 + G2s;
-          const double K = P * f;
-          const double e = K * g;
-          double sx, sy, sz;
-          if (FULL) {
-            sx = sy = sz = 2.0;
-          } else {
-            const double mxz = bb <= c3sq ? 1.0 : 0.0, myz = cc <= c3sq ? 1.0 : 0.0;
-            const double m2 = mxz * myz;
-            sx = fma(myz, mxy, m2);
-            sy = fma(mxz, mxy, m2);
-            sz = mxy * (mxz + myz);
-          }
-          if (ENERGY) {
-            if (FULL) {
-              ex += e;  // x and y alike
-              rE += e;
-            } else {
-              ex = fma(sx, e, ex);
-              ey = fma(sy, e, ey);
-              rE = fma(sz, e, rE);
-            }
-          }
-          if (GRAD) {
-            const double W6 = FULL ? 2.0 * (gxy + rz.w) : fma(gx, sx, fma(gy, sy, rz.w * sz));
-            const double We = W6 * e;
-            const double H = fma(g, fma(alp * T, f, -1.5), -G2s);
-            // ds/da, ds/db, ds/dc from A = Y W3, B = X W3, C = X Y
-            const double A = Y * W3, Bq = X * W3, C = X * Y;
-            const double Ssum = A + Bq + C;
-            const double KW = K * W6;  // half of the dL/de-weighted factor; x2 at the end
-            const double al = KW * H, be = KW * G2;
-            const double ali = al * a * iabc;  // al / (b c)
-            const double ca = fma(al, ia, be * fma(-2.0, C, Ssum));
-            const double cb = fma(ali, cc, be * fma(-2.0, Bq, Ssum));
-            const double cg = fma(ali, bb, be * fma(-2.0, A, Ssum));
-            // x gets ca (x - y) + cb (x - z), y gets cg (y - z) - ca (x - y):
-            // the (x - y) part is summed as a scalar
-            sca += ca;
-            fx0 = fma(cb, u0, fx0);
-            fx1 = fma(cb, u1, fx1);
-            fx2 = fma(cb, u2, fx2);
-            fy0 = fma(cg, v0, fy0);
-            fy1 = fma(cg, v1, fy1);
-            fy2 = fma(cg, v2, fy2);
-            rF0 = fma(-cb, u0, fma(-cg, v0, rF0));
-            rF1 = fma(-cb, u1, fma(-cg, v1, rF1));
-            rF2 = fma(-cb, u2, fma(-cg, v2, rF2));
-            cx = fma(We, kxy + x23.y, cx);
-            cy = fma(We, kyx + y23.y, cy);
-            rD = fma(We, x23.x + y23.x, rD);
-            if (PG) {
-              const double TdT = -6.0 * f * We * T;  // dL6/dT times T
-              g_s9 = fma(W6, P0 * f * g, g_s9);
-              g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
-              const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
-              const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
-              g_a1 = fma(TdT * alp / 3.0, Axy + sxz * bxz + syz * byz, g_a1);
-              g_a2 = fma(TdT * alp / 3.0, Bxy + bxz + byz, g_a2);
-            }
-          }
-          // hand the sums for z to the lane that takes z at the next step:
-          // k ^ (k + 1) is 1, 3, 1, 7, 1, 3, 1 (only the last step of each
-          // group of four depends on the group)
-          if ((k & 1) == 0)
-            ring_xor8<1>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
-          else if ((k & 3) == 1)
-            ring_xor8<3>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
-          else if (k == 3)
-            ring_xor8<7>(rE, rD, rF0, rF1, rF2, ENERGY, GRAD);
-        }
-      };
+      Atm8Ring rr = {0.0, 0.0, 0.0, 0.0, 0.0};
       const bool first = z0 == by * T8;
       if (full) {
         if (first)
-          steps(std::true_type{}, std::true_type{});
+          atm8_chunk<ENERGY, GRAD, PG, true, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y);
         else
-          steps(std::true_type{}, std::false_type{});
+          atm8_chunk<ENERGY, GRAD, PG, true, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y);
       } else {
         if (first)
-          steps(std::false_type{}, std::true_type{});
+          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y);
         else
-          steps(std::false_type{}, std::false_type{});
+          atm8_chunk<ENERGY, GRAD, PG, false, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0,
+                                                     y);
       }
-      if (full) rE *= 2.0;
+      if (full) rr.E *= 2.0;
       // lane tx of every row now holds that row's sums for z0 + (tx ^ 7) (the
       // gradient parts at half weight); add the 8 rows in order
-      s_red[0][lane] = rE;
-      s_red[1][lane] = 2.0 * rF0;
-      s_red[2][lane] = 2.0 * rF1;
-      s_red[3][lane] = 2.0 * rF2;
-      s_red[4][lane] = rD;
+      atm8_stage_sums<NA>(s_red, lane, rr.E, rr.F0, rr.F1, rr.F2, rr.D);
       __syncthreads();
       if (lane < T8 * NA) {
         const int l = lane & (T8 - 1), c = lane >> 3;  // column l holds z0 + (l ^ 7)
@@ -1572,21 +1665,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     }
     // tile sums of the x atoms (over the rows) and the y atoms (over the columns)
     if (full) {
-      ex *= 2.0;
-      ey = ex;
+      u.ex *= 2.0;
+      u.ey = u.ex;
     }
-    fx0 = fma(sca, dxy0, fx0);
-    fx1 = fma(sca, dxy1, fx1);
-    fx2 = fma(sca, dxy2, fx2);
-    fy0 = fma(-sca, dxy0, fy0);
-    fy1 = fma(-sca, dxy1, fy1);
-    fy2 = fma(-sca, dxy2, fy2);
+    atm8_fold_sca(q, u);
     __syncthreads();
-    s_red[0][lane] = ex;
-    s_red[1][lane] = 2.0 * fx0;
-    s_red[2][lane] = 2.0 * fx1;
-    s_red[3][lane] = 2.0 * fx2;
-    s_red[4][lane] = cx;
+    atm8_stage_sums<NA>(s_red, lane, u.ex, u.fx0, u.fx1, u.fx2, u.cx);
     __syncthreads();
     if (lane < T8 * NA) {
       const int col = lane & (T8 - 1), c = lane >> 3;
@@ -1597,17 +1681,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
       }
     }
     __syncthreads();
-    s_red[0][lane] = ey;
-    s_red[1][lane] = 2.0 * fy0;
-    s_red[2][lane] = 2.0 * fy1;
-    s_red[3][lane] = 2.0 * fy2;
-    s_red[4][lane] = cy;
+    atm8_stage_sums<NA>(s_red, lane, u.ey, u.fy0, u.fy1, u.fy2, u.cy);
     __syncthreads();
     if (lane < T8 * NA) {
       const int row = lane & (T8 - 1), c = lane >> 3;
       if ((c == 0 ? ENERGY : GRAD) && by * T8 + row < n) {
         double sum = 0.0;
-        for (int q = 0; q < T8; ++q) sum += s_red[c][row * T8 + q];
+        for (int q2 = 0; q2 < T8; ++q2) sum += s_red[c][row * T8 + q2];
         s_acc[c * n + by * T8 + row] += sum;
       }
     }
@@ -1618,12 +1698,257 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
     const int c = e / n, i = e - c * n;
     part[c * N + i] = s_acc[e];
   }
-  if (PG) {
-    const double sixth = 1.0 / 6.0;
-    const double v4[4] = {g_s9 * sixth, g_alp * sixth, g_a1 * sixth, g_a2 * sixth};
-    for (int k = 0; k < 4; ++k) {
-      const double v = wave_sum(v4[k]);
-      if (lane == 0) w.pg3[(m * (int64_t)S + s) * 4 + k] = v;
+  if (PG) atm8_store_pgrad(u, lane, w.pg3 + (m * (int64_t)S + s) * 4);
+}
+
+// --------------------------------------------------------------------------
+// One-pass three-body kernel for large systems (the atoms in spatial order):
+// the 8 x 8 evaluation above with group screening. Work item (m, gx, s)
+// owns the x atoms of 8-atom group gx and takes every S-th of the pair tiles
+// (gx, gy), gy >= gx, whose groups can hold a counted triple (bounding
+// spheres within 2 c3); per tile it streams the third-atom groups gz >= gy
+// with at least two of the three group-to-group gaps within c3. Screened-out
+// triples contribute exact zeros, so the screening changes no bit of the
+// result beyond the order of the sums.
+//
+// Outputs: each item adds its sums for the atoms of group g (x, y or z) to
+// its own slot of g in the window buffer: slots of group g are contiguous,
+// one per item (gx <= g, s), in item order, so k_atm8s_rows adds them in a
+// fixed order. The number of splits S of x-group gx depends on the
+// molecule's atom count alone (more for the first groups, whose forward
+// neighbourhood is largest), and heavy items are dispatched first.
+// --------------------------------------------------------------------------
+
+// Slot tables of molecule m: [0] = largest split count, [1 + g] = first slot
+// of items of x-group g in the list of any later group (g = 0 .. GN), then
+// [GN + 2 + g] = first slot of group g in the window buffer (g = 0 .. GN).
+__device__ __forceinline__ const int64_t* atm8s_table(const Work& w, int64_t m) {
+  return w.wix + m * atm8s_table_len(w.N);
+}
+
+// Build the slot tables of every molecule (one thread per molecule: O(G)).
+__global__ __launch_bounds__(64) void k_atm8s_index(Work w) {
+  const int64_t m = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (m >= w.B) return;
+  const int n = w.natom[m];
+  const int G = (n + kAtm8 - 1) / kAtm8;
+  const int64_t GN = n_groups8(w.N);
+  int64_t* t = w.wix + m * atm8s_table_len(w.N);
+  const int smax = atm8s_smax(n);
+  t[0] = smax;
+  int64_t pre = 0, base = 0;
+  for (int g = 0; g <= GN; ++g) {
+    t[1 + g] = pre;
+    t[GN + 2 + g] = base;
+    if (g < G) {
+      pre += atm8s_splits(G, g, smax);
+      base += pre;  // group g has one slot per item of x-groups 0 .. g
+    }
+  }
+}
+
+template <bool ENERGY, bool GRAD, bool PG>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_atm8s(
+    Params p, Inputs in, Work w, int64_t nblocks) {
+  constexpr int NA = kAtm1Acc;
+  constexpr int T8 = kAtm8;
+  constexpr int KS = kAtm8sMaxSplits;
+  constexpr int kWin = NA * T8;  // doubles per slot: [NA][8 atoms]
+  __shared__ __align__(16) double s_zc[T8 * kAtm8Row];
+  __shared__ __align__(16) double s_r[T8 * 6];
+  __shared__ double s_rt[T8];
+  __shared__ double s_red[NA][64];
+
+  const int64_t b = blockIdx.x;
+  if (b >= nblocks) return;
+  const int lane = threadIdx.x, tx = lane & (T8 - 1), ty = lane >> 3;
+  const int64_t N = w.N;
+  const int64_t per_mol = atm8s_blocks_per_mol(N);
+  const int64_t m = b / per_mol;
+  // block r of a molecule runs on XCD r % 8: x-group gx stays on XCD gx % 8
+  // with all its splits, and the heavy low groups come first
+  const int r = (int)(b - m * per_mol);
+  const int s = (r >> 3) % KS, gx = ((r >> 3) / KS) * 8 + (r & 7);
+  const int n = w.natom[m];
+  const int G = (n + T8 - 1) / T8;
+  const int64_t* tab = atm8s_table(w, m);
+  const int smax = (int)tab[0];
+  const int S = gx < G ? atm8s_splits(G, gx, smax) : 0;
+  if (s >= S) {
+    if (PG && lane < 4) w.pg3[b * 4 + lane] = 0.0;
+    return;
+  }
+  const int64_t GN = n_groups8(N);
+  const int64_t slot = tab[1 + gx] + s;  // this item's slot in the list of every group >= gx
+  const int64_t* gbase = tab + GN + 2;
+  double* win = w.pg + m * atm8s_stride(N);
+  auto slot_of = [&](int g) { return win + (gbase[g] + slot) * kWin; };
+  for (int g = gx; g < G; ++g)
+    if (lane < kWin) slot_of(g)[lane] = 0.0;
+  __syncthreads();
+
+  const int64_t* Z = in.numbers + m * N;
+  const double* S8 = w.gsph8 + 4 * m * GN;
+  const double* SX = S8 + 4 * gx;
+  const double c3 = sqrt(p.cut3sq) * (1.0 + 1e-10) + 1e-10;  // margin for rounding in the bounds
+  const int x = gx * T8 + tx;
+  Atm8Sums u = {};
+  int kept = 0;  // pair tiles of this x-group seen so far
+  for (int ybase = gx; ybase < G; ybase += 64) {
+    const int cand = ybase + lane;
+    const bool ykeep = cand < G && sphere_gap(SX, S8 + 4 * cand) <= 2.0 * c3;
+    unsigned long long ymask = __ballot(ykeep);
+    while (ymask) {
+      const int gy = ybase + __builtin_ctzll(ymask);
+      ymask &= ymask - 1;
+      if (kept++ % S != s) continue;
+      const double* SY = S8 + 4 * gy;
+      const int nxy = sphere_gap(SX, SY) <= c3 ? 1 : 0;
+      const int y = gy * T8 + ty;
+      const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
+      const Atm8Pair q = atm8_pair<GRAD, PG>(p, in, w, m, x, y, valid);
+      u.ey = u.cy = u.sca = 0.0;
+      u.fy0 = u.fy1 = u.fy2 = 0.0;
+
+      // third-atom groups in ascending order, 64 candidates per ballot
+      int zbase = gy;
+      auto zscan = [&]() {
+        const int c = zbase + lane;
+        bool k = false;
+        if (c < G)
+          k = nxy + (sphere_gap(SX, S8 + 4 * c) <= c3 ? 1 : 0) +
+                  (sphere_gap(SY, S8 + 4 * c) <= c3 ? 1 : 0) >=
+              2;
+        return (unsigned long long)__ballot(k);
+      };
+      unsigned long long zmask = zscan();
+      auto next_z = [&]() {
+        while (true) {
+          if (zmask) {
+            const int g = zbase + __builtin_ctzll(zmask);
+            zmask &= zmask - 1;
+            return g;
+          }
+          zbase += 64;
+          if (zbase >= G) return G;
+          zmask = zscan();
+        }
+      };
+      Atm8Chunk ch;
+      int gz = next_z();
+      if (gz < G) atm8_fetch<GRAD, PG>(ch, in, w, m, n, gz * T8, gx * T8, gy * T8, lane);
+      while (gz < G) {
+        __syncthreads();
+        atm8_stage<PG>(ch, s_zc, s_r, s_rt, lane);
+        __syncthreads();
+        const int gzn = next_z();
+        if (gzn < G)  // in flight during this chunk
+          atm8_fetch<GRAD, PG>(ch, in, w, m, n, gzn * T8, gx * T8, gy * T8, lane);
+        // lane c * 8 + l adds the sums of component c for z = 8 gz + (l ^ 7)
+        double* wz = slot_of(gz);
+        const int widx = (lane & ~(T8 - 1)) | ((lane & (T8 - 1)) ^ 7);
+        const double old = lane < kWin ? wz[widx] : 0.0;
+        Atm8Ring rr = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (gz == gy)
+          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty,
+                                                    gz * T8, y);
+        else
+          atm8_chunk<ENERGY, GRAD, PG, false, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty,
+                                                     gz * T8, y);
+        atm8_stage_sums<NA>(s_red, lane, rr.E, rr.F0, rr.F1, rr.F2, rr.D);
+        __syncthreads();
+        if (lane < kWin) {
+          const int l = lane & (T8 - 1), c = lane >> 3;
+          if (c == 0 ? ENERGY : GRAD) {
+            double sum = 0.0;
+            for (int r2 = 0; r2 < T8; ++r2) sum += s_red[c][r2 * T8 + l];
+            wz[widx] = old + sum;
+          }
+        }
+        gz = gzn;
+      }
+      // the y atoms' tile sums (over the columns)
+      atm8_fold_sca(q, u);
+      __syncthreads();
+      atm8_stage_sums<NA>(s_red, lane, u.ey, u.fy0, u.fy1, u.fy2, u.cy);
+      __syncthreads();
+      if (lane < kWin) {
+        const int row = lane & (T8 - 1), c = lane >> 3;
+        if (c == 0 ? ENERGY : GRAD) {
+          double sum = 0.0;
+          for (int q2 = 0; q2 < T8; ++q2) sum += s_red[c][row * T8 + q2];
+          double* wy = slot_of(gy);
+          wy[lane] += sum;
+        }
+      }
+    }
+  }
+  // the x atoms' sums over all tiles (over the rows)
+  __syncthreads();
+  atm8_stage_sums<NA>(s_red, lane, u.ex, u.fx0, u.fx1, u.fx2, u.cx);
+  __syncthreads();
+  if (lane < kWin) {
+    const int col = lane & (T8 - 1), c = lane >> 3;
+    if (c == 0 ? ENERGY : GRAD) {
+      double sum = 0.0;
+      for (int r2 = 0; r2 < T8; ++r2) sum += s_red[c][r2 * T8 + col];
+      slot_of(gx)[lane] += sum;
+    }
+  }
+  if (PG) atm8_store_pgrad(u, lane, w.pg3 + b * 4);
+}
+
+// Atom sums of the screened one-pass kernel: one work-group per 8-atom group
+// adds the group's slots in slot order (thread (j, v) takes slots j, j + J,
+// ..., then the J partial sums are added in order).
+template <bool ENERGY, bool GRAD>
+__global__ __launch_bounds__(1024) void k_atm8s_rows(Inputs in, Work w, double* energy_out,
+                                                     double* grad_out) {
+  constexpr int NA = kAtm1Acc, T8 = kAtm8, kWin = NA * T8, J = 1024 / kWin;
+  __shared__ double s_sum[J][kWin];
+  const int64_t N = w.N;
+  const int64_t GN = n_groups8(N);
+  const int64_t m = blockIdx.x / GN;
+  const int g = (int)(blockIdx.x - m * GN);
+  const int n = w.natom[m];
+  const int G = (n + T8 - 1) / T8;
+  const int tid = threadIdx.x;
+  const int j = tid / kWin, v = tid - j * kWin;
+  double acc = 0.0;
+  if (g < G && j < J) {
+    const int64_t* tab = atm8s_table(w, m);
+    const int64_t cnt = tab[1 + g + 1];  // items of x-groups 0 .. g
+    const double* src = w.pg + m * atm8s_stride(N) + tab[GN + 2 + g] * kWin + v;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int64_t k = j;
+    for (; k + 3 * J < cnt; k += 4 * J) {
+      a0 += src[k * kWin];
+      a1 += src[(k + J) * kWin];
+      a2 += src[(k + 2 * J) * kWin];
+      a3 += src[(k + 3 * J) * kWin];
+    }
+    for (; k < cnt; k += J) a0 += src[k * kWin];
+    acc = (a0 + a1) + (a2 + a3);
+  }
+  if (j < J) s_sum[j][v] = acc;
+  __syncthreads();
+  if (tid < kWin) {
+    double t = 0.0;
+    for (int q = 0; q < J; ++q) t += s_sum[q][tid];
+    const int c = tid >> 3, l = tid & (T8 - 1);
+    const int i = g * T8 + l;
+    if (i < N) {
+      const int64_t row = m * N + i;
+      const bool real = i < n && in.numbers[row] != 0;
+      const double sixth = 1.0 / 6.0;
+      if (c == 0) {
+        if (ENERGY) energy_out[row] = real ? w.e2[row] + t * sixth : 0.0;
+      } else if (GRAD && real) {
+        if (c <= 3)
+          grad_out[3 * row + c - 1] += t * sixth;
+        else
+          w.dldcn[row] += t * sixth;
+      }
     }
   }
 }
@@ -1773,7 +2098,10 @@ __global__ __launch_bounds__(256) void k_copy(const double* src, double* dst,
 
 // Parameter gradients: fixed-order sums of the two-body row partials and the
 // three-body work-group partials into dL/d(s6, s8, s10, s9, a1, a2, alp).
-__global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int one_pass,
+// mode: 0 pair-tile kernel (one partial per 16 x 16 pair tile), 1 one-pass
+// kernel (one per split; tile8 picks the 8 x 8 variant), 2 screened one-pass
+// kernel (one per work-group, all written).
+__global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int mode,
                                                       int tile8, int64_t per_mol, double* out) {
   __shared__ double red[2 * kWaves];
   const int tid = threadIdx.x;
@@ -1792,7 +2120,9 @@ __global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int one
     const int k = (int)(q - m * per_mol);
     const int n = w.natom[m];
     const int nb = (n + kTile - 1) / kTile;
-    const int valid = one_pass ? one_pass_splits(n, tile8 != 0) : nb * (nb + 1) / 2;
+    const int64_t valid = mode == 2   ? per_mol
+                          : mode == 1 ? one_pass_splits(n, tile8 != 0)
+                                      : nb * (nb + 1) / 2;
     if (n == 0 || k >= valid) continue;
     const double* g = w.pg3 + q * 4;
     a[3] += g[0];
@@ -1926,9 +2256,29 @@ hipError_t launch_atm1(hipStream_t s, const Params& p, const Inputs& in, const W
   return hipGetLastError();
 }
 
+hipError_t launch_atm8s(hipStream_t s, const Params& p, const Inputs& in, const Work& w,
+                        bool energy, bool grad) {
+  const int64_t nblocks = w.B * atm8s_blocks_per_mol(w.N);
+  if (nblocks == 0) return hipSuccess;
+  hipError_t e = launch_groups8(s, in, w);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_atm8s_index, dim3((unsigned)((w.B + 63) / 64)), dim3(64), 0, s, w);
+  const dim3 g((unsigned)nblocks), blk(64);
+  if (energy && grad)
+    hipLaunchKernelGGL((k_atm8s<true, true, false>), g, blk, 0, s, p, in, w, nblocks);
+  else if (energy)
+    hipLaunchKernelGGL((k_atm8s<true, false, false>), g, blk, 0, s, p, in, w, nblocks);
+  else if (grad && p.pgrad)
+    hipLaunchKernelGGL((k_atm8s<false, true, true>), g, blk, 0, s, p, in, w, nblocks);
+  else if (grad)
+    hipLaunchKernelGGL((k_atm8s<false, true, false>), g, blk, 0, s, p, in, w, nblocks);
+  return hipGetLastError();
+}
+
 hipError_t launch_atm(hipStream_t s, const Params& p, const Inputs& in,
                       const Work& w, bool energy, bool grad) {
   if (one_pass(p, w)) return launch_atm1(s, p, in, w, energy, grad);
+  if (!p.pair_owner) return launch_atm8s(s, p, in, w, energy, grad);
   const int nb = (int)((w.N + kTile - 1) / kTile);
   const int tiles = nb * (nb + 1) / 2;
   const int64_t ntiles = w.B * tiles;
@@ -1968,6 +2318,19 @@ hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, con
                          energy_out, grad_out);
     return hipGetLastError();
   }
+  if (!p.pair_owner) {
+    const dim3 grid((unsigned)(w.B * n_groups8(w.N))), blk(1024);
+    if (energy && grad)
+      hipLaunchKernelGGL((k_atm8s_rows<true, true>), grid, blk, 0, s, in, w, energy_out,
+                         grad_out);
+    else if (energy)
+      hipLaunchKernelGGL((k_atm8s_rows<true, false>), grid, blk, 0, s, in, w, energy_out,
+                         grad_out);
+    else if (grad)
+      hipLaunchKernelGGL((k_atm8s_rows<false, true>), grid, blk, 0, s, in, w, energy_out,
+                         grad_out);
+    return hipGetLastError();
+  }
   if (energy && grad)
     hipLaunchKernelGGL((k_atm_rows<true, true>), dim3(rows_grid(w)), dim3(256), 0, s,
                        in, w, energy_out, grad_out);
@@ -1998,8 +2361,11 @@ hipError_t launch_pgrad_reduce(hipStream_t s, const Params& p, const Inputs& in,
   const bool op = one_pass(p, w);
   const bool tile8 = one_pass_tile8(p, true);
   const int64_t nb = n_groups(w.N);
-  const int64_t per_mol = op ? one_pass_splits((int)w.N, tile8) : nb * (nb + 1) / 2;
-  hipLaunchKernelGGL(k_pgrad_reduce, dim3(1), dim3(256), 0, s, in, w, op ? 1 : 0, tile8 ? 1 : 0,
+  const int mode = op ? 1 : (p.pair_owner ? 0 : 2);
+  const int64_t per_mol = mode == 1   ? one_pass_splits((int)w.N, tile8)
+                          : mode == 2 ? atm8s_blocks_per_mol(w.N)
+                                      : nb * (nb + 1) / 2;
+  hipLaunchKernelGGL(k_pgrad_reduce, dim3(1), dim3(256), 0, s, in, w, mode, tile8 ? 1 : 0,
                      per_mol, out);
   return hipGetLastError();
 }

@@ -101,6 +101,7 @@ class KernelParams:
     force_pair_tiles: bool = False
     d4s: bool = False  # D4S model (pairwise Gaussian weighting factors)
     atm16: bool = False  # testing: one-pass three-body kernel with 16 x 16 pair tiles
+    pair_owner: bool = False  # testing: large systems take the pair-owner three-body kernel
 
     def c(self) -> _lib.Params:
         flags = (
@@ -111,6 +112,7 @@ class KernelParams:
             | (_lib.D4_FLAG_FORCE_PAIR_TILES if self.force_pair_tiles else 0)
             | (_lib.D4_FLAG_D4S if self.d4s else 0)
             | (_lib.D4_FLAG_ATM16 if self.atm16 else 0)
+            | (_lib.D4_FLAG_PAIR_OWNER if self.pair_owner else 0)
         )
         return _lib.Params(
             self.s6, self.s8, self.s10, self.s9, self.a1, self.a2, self.alp,

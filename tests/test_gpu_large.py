@@ -151,8 +151,9 @@ def test_screened_cluster_n300_vs_oracle():
 def test_cluster_5000_invariants():
     """The 5000-atom cluster at its real size: too large for the dense oracle
     (n^3 tensors), so size-independent properties are checked: translation
-    invariance, zero net force, and central finite differences of the total
-    energy on three coordinates."""
+    invariance, zero net force, agreement of the two large-system three-body
+    kernels, and central finite differences of the total energy on three
+    coordinates."""
     _say("cluster_5000: translation, net force, finite differences")
     mol = synth.workload("cluster_5000")[0]
     kp = _kp()
@@ -165,6 +166,10 @@ def test_cluster_5000_invariants():
     assert abs(e2.sum().item() - etot) < 1e-11 * abs(etot)
     assert (g2 - g).abs().max().item() < 1e-11
     assert g.sum(dim=1).abs().max().item() < 1e-9
+    # the pair-owner three-body kernel (each triple from its three pairs)
+    e3, g3, _ = runtime.energy_gradient(nt, pt, ct, _kp(pair_owner=True))
+    assert (e3 - e).abs().max().item() < 1e-13
+    assert (g3 - g).abs().max().item() < 1e-12
     h = 1e-4
     for atom, d in ((17, 0), (2500, 1), (4999, 2)):
         p1 = pt.clone()
@@ -177,17 +182,21 @@ def test_cluster_5000_invariants():
         assert abs(fd - g[0, atom, d].item()) < 1e-8, (atom, d, fd, g[0, atom, d].item())
 
 
-def test_one_pass_three_body_matches_pair_tiles():
+@pytest.mark.parametrize("pair_owner", [False, True])
+def test_one_pass_three_body_matches_pair_tiles(pair_owner):
     """The one-pass three-body kernel (each triple once, molecules up to 256
-    atoms) against the pair-tile kernel (each triple from all three pairs):
-    energies, fused gradient, and a weighted split backward; sizes cross the
-    16-atom tile edges, short cutoffs exercise the screening masks."""
-    _say("one-pass three-body vs pair tiles, n = 5, 16, 17, 40, 77, 130")
+    atoms) against the large-system kernels forced at these sizes: the
+    screened one-pass kernel (8-atom groups, window slots) and the pair-owner
+    kernel (each triple from all three pairs). Energies, fused gradient, and
+    a weighted split backward; sizes cross the 8- and 16-atom group edges,
+    short cutoffs exercise the screening masks."""
+    _say(f"one-pass three-body vs large-system kernel (pair owner {pair_owner}), "
+         "n = 5, 16, 17, 40, 77, 130")
     mols = [_mol(120 + k, n) for k, n in enumerate((5, 16, 17, 40, 77, 130))]
     numbers, positions, charge = synth.pack(mols, device="cuda")
     for cut in (_cut(), d4.Cutoff(disp2=30.0, disp3=7.0, cn=25.0, cn_eeq=12.0)):
         one = _kp(cut)
-        ref = _kp(cut, force_pair_tiles=True)
+        ref = _kp(cut, force_pair_tiles=True, pair_owner=pair_owner)
         e1, g1, _ = runtime.energy_gradient(numbers, positions, charge, one)
         e2, g2, _ = runtime.energy_gradient(numbers, positions, charge, ref)
         assert (e1 - e2).abs().max().item() < 1e-13
@@ -197,9 +206,48 @@ def test_one_pass_three_body_matches_pair_tiles():
         assert (eo - e1).abs().max().item() < 1e-14
         assert (eo - ep).abs().max().item() < 1e-13
         gE = torch.rand_like(eo)
-        ga = runtime.backward(numbers, positions, charge, one, ws1, grad_energy=gE)
-        gb = runtime.backward(numbers, positions, charge, ref, ws2, grad_energy=gE)
+        ga, pa = runtime.backward(numbers, positions, charge, one, ws1, grad_energy=gE,
+                                  params=True)
+        gb, pb = runtime.backward(numbers, positions, charge, ref, ws2, grad_energy=gE,
+                                  params=True)
         assert (ga - gb).abs().max().item() < 1e-12
+        assert torch.allclose(pa, pb, rtol=1e-11, atol=1e-14)
+
+
+def test_screened_one_pass_matches_pair_owner():
+    """The two large-system three-body kernels on a batch that takes the
+    large-system path by itself (a 300-atom cluster padded with a 90- and a
+    5-atom molecule): the screened one-pass kernel (default) against the
+    pair-owner kernel, default and short cutoffs (group screening skips
+    pair tiles and third-atom groups), fused, split and parameter
+    gradients."""
+    _say("screened one-pass vs pair-owner three-body, n = 300, 90, 5")
+    mols = [_mol(160, 300, mix=synth.MIX_HCNOS, spacing=2.6, charges=(0,)), _mol(161, 90),
+            _mol(162, 5)]
+    numbers, positions, charge = synth.pack(mols, device="cuda")
+    for cut in (_cut(), d4.Cutoff(disp2=20.0, disp3=8.0, cn=25.0, cn_eeq=10.0)):
+        a = _kp(cut)
+        b = _kp(cut, pair_owner=True)
+        e1, g1, _ = runtime.energy_gradient(numbers, positions, charge, a)
+        e2, g2, _ = runtime.energy_gradient(numbers, positions, charge, b)
+        assert (e1 - e2).abs().max().item() < 1e-13
+        assert (g1 - g2).abs().max().item() < 1e-12
+        ea, ws1 = runtime.forward(numbers, positions, charge, a)
+        eb, ws2 = runtime.forward(numbers, positions, charge, b)
+        assert (ea - e1).abs().max().item() < 1e-14
+        assert (ea - eb).abs().max().item() < 1e-13
+        gE = torch.rand_like(ea)
+        ga, pa = runtime.backward(numbers, positions, charge, a, ws1, grad_energy=gE,
+                                  params=True)
+        gb, pb = runtime.backward(numbers, positions, charge, b, ws2, grad_energy=gE,
+                                  params=True)
+        assert (ga - gb).abs().max().item() < 1e-12
+        assert torch.allclose(pa, pb, rtol=1e-11, atol=1e-14)
+        # padding atoms stay zero
+        assert float(e1[2, 5:].abs().sum()) == 0.0 and float(g1[2, 5:].abs().sum()) == 0.0
+        # bitwise repeatable
+        e3, g3, _ = runtime.energy_gradient(numbers, positions, charge, a)
+        assert torch.equal(e1, e3) and torch.equal(g1, g3)
 
 
 def test_one_pass_8x8_matches_16x16():

@@ -75,6 +75,7 @@ def _weights(mols, seed):
 
 
 @pytest.mark.parametrize("flags", [{}, {"force_pair_tiles": True},
+                                   {"force_pair_tiles": True, "pair_owner": True},
                                    {"force_sort": True, "force_tiled": True}])
 def test_param_gradients_vs_oracle(flags):
     _say(f"parameter gradients vs oracle autograd, flags={flags}")
@@ -92,7 +93,8 @@ def test_param_gradients_screened_vs_oracle():
     mols = [_mol(210, 64, spacing=2.6), _mol(211, 25)]
     wts = _weights(mols, 6)
     ref = _oracle_grads(mols, wts, cut)
-    for flags in ({}, {"force_pair_tiles": True}):
+    for flags in ({}, {"force_pair_tiles": True},
+                  {"force_pair_tiles": True, "pair_owner": True}):
         got = _gpu_grads(mols, wts, cut, **flags)
         assert torch.allclose(got, ref, rtol=1e-9, atol=1e-12), (flags, got - ref)
 
