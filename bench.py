@@ -231,7 +231,9 @@ def main() -> int:
         step()
     torch.cuda.synchronize()
     runtime.profile_read(dev)  # clear
-    runtime.profile_enable(dev, True)
+    # timed region: HIP events on the three-body kernel's launch stream
+    # bracket that kernel alone (events around every kernel add ~5 % here)
+    runtime.profile_enable(dev, True, slots=("atm",))
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -245,12 +247,18 @@ def main() -> int:
     prof = runtime.profile_read(dev)
     ms_step = ms_total / args.steps
     value = B * args.steps / (ms_total / 1e3)
+    # per-kernel averages from a second, fully instrumented pass of the same steps
+    runtime.profile_enable(dev, True)
+    for _ in range(args.steps):
+        step()
+    runtime.profile_enable(dev, False)
+    prof_all = runtime.profile_read(dev)
 
     work = metrics.workload_flops(mols, CUT[0], CUT[1], with_forces=forces)
     atm_ms, atm_n = prof["atm"]
     atm_avg_s = atm_ms / max(atm_n, 1) / 1e3
     achieved = work["atm"] / atm_avg_s / 1e12
-    kernel_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items() if v[1]}
+    kernel_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in prof_all.items() if v[1]}
 
     out = {
         "metric": METRICS[args.workload],
@@ -300,6 +308,9 @@ def main() -> int:
                                        / metrics.PEAK_FP64_TFLOPS, 4),
         },
         "kernel_avg_ms": kernel_ms,
+        "kernel_avg_note": "per-kernel HIP-event averages from a second pass of the same steps "
+                           "with every kernel bracketed; the timed region brackets only the "
+                           "three-body kernel",
     }
     if args.workload == "cluster_5000":
         out["unit"] = "evaluations/s"

@@ -161,9 +161,10 @@ int d4_molecule_status(void* stream, int64_t nbatch, const void* workspace,
 
 /* Diagnostics (no counterpart in the reference): per-kernel HIP-event
  * timing on the launch stream. While enabled, every launch made through
- * `ctx` is bracketed by events; d4_profile_read synchronises the context's
- * last stream, adds the elapsed times up per kernel slot (D4_SLOT_*) and
- * resets the accumulators. */
+ * `ctx` (on = 1), or every launch of the slots set in a mask (on =
+ * D4_PROFILE_SLOTS | (1 << slot) | ...), is bracketed by events;
+ * d4_profile_read synchronises the recorded events, adds the elapsed times
+ * up per kernel slot (D4_SLOT_*) and resets the accumulators. */
 #define D4_SLOT_PREP 0
 #define D4_SLOT_CN 1
 #define D4_SLOT_EEQ 2
@@ -174,6 +175,7 @@ int d4_molecule_status(void* stream, int64_t nbatch, const void* workspace,
 #define D4_SLOT_ADJOINT 7
 #define D4_SLOT_CHAIN 8
 #define D4_NUM_SLOTS 9
+#define D4_PROFILE_SLOTS (1 << 16)
 int d4_profile_enable(d4_context* ctx, int on);
 int d4_profile_read(d4_context* ctx, double* ms_out, int64_t* count_out, int nslots);
 

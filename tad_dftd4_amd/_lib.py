@@ -31,6 +31,7 @@ D4_FLAG_ATM16 = 64
 D4_FLAG_PAIR_OWNER = 128
 
 SLOTS = ("prep", "cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain")
+D4_PROFILE_SLOTS = 1 << 16
 
 STATUS_LU = 1
 STATUS_SINGULAR = 2

@@ -33,7 +33,7 @@ struct d4_context {
   std::mutex aux_mu;
   std::unordered_map<hipStream_t, d4_aux> aux;
   // diagnostics: per-kernel event timing
-  bool profile = false;
+  unsigned profile = 0;  // bit k: bracket launches of slot k with events
   std::vector<d4_event_pair> pending;
   std::vector<hipEvent_t> pool;
   double prof_ms[D4_NUM_SLOTS] = {};
@@ -179,7 +179,7 @@ hipEvent_t take_event(d4_context* ctx) {
 // Launch through `fn`, bracketed by timing events when profiling is on.
 template <class F>
 hipError_t timed(d4_context* ctx, hipStream_t s, int slot, F&& fn) {
-  if (!ctx->profile) return fn();
+  if (!((ctx->profile >> slot) & 1u)) return fn();
   hipEvent_t a = take_event(ctx), b = take_event(ctx);
   if (!a || !b) return fn();
   hipError_t e = hipEventRecord(a, s);
@@ -535,7 +535,12 @@ int d4_molecule_status(void* stream, int64_t nbatch, const void* workspace,
 
 int d4_profile_enable(d4_context* ctx, int on) {
   if (ctx == nullptr) return set_error(D4_ERR_INVALID, "context is NULL");
-  ctx->profile = on != 0;
+  if (on == 0 || on == 1)
+    ctx->profile = on ? (1u << D4_NUM_SLOTS) - 1u : 0u;
+  else if (on & D4_PROFILE_SLOTS)
+    ctx->profile = (unsigned)on & ((1u << D4_NUM_SLOTS) - 1u);
+  else
+    return set_error(D4_ERR_INVALID, "profile: 0, 1 or D4_PROFILE_SLOTS | slot bits");
   return D4_OK;
 }
 

@@ -232,6 +232,58 @@ def properties(numbers, positions, charge, kp: KernelParams, ws=None):
     return cn, q, c6, alpha
 
 
+class CapturedStep:
+    """One evaluation of a fixed-shape batch captured into a HIP graph and
+    replayed: the launch sequence of the caller's stream and of its companion
+    stream (forked and joined by events) without the host-side launch gaps.
+
+    The inputs are copied into static device buffers (``numbers``,
+    ``positions``, ``charge``, and ``q``/``r4r2`` when given); write new values
+    into them in place between replays. Damping parameters and cutoffs are
+    fixed at capture. ``replay()`` enqueues the graph on the current stream
+    and returns the static outputs: the atom energies, with ``forces`` also
+    the gradient (the same values as ``energy_gradient``/``forward``)."""
+
+    def __init__(self, numbers, positions, charge, kp: KernelParams, forces: bool = True,
+                 q=None, r4r2=None):
+        _check_inputs(numbers, positions, charge)
+        dev = positions.device
+        B, N = numbers.shape
+        self.kp = kp
+        self.forces = forces
+        self.numbers = numbers.clone()
+        self.positions = positions.clone()
+        self.charge = charge.clone()
+        self.q = None if q is None else q.clone()
+        self.r4r2 = None if r4r2 is None else r4r2.clone()
+        self.ws = new_workspace(B, N, dev)
+        self.energy = torch.empty((B, N), dtype=torch.float64, device=dev)
+        self.grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev) if forces else None
+        # capture on a stream of our own; one eager call on it first creates
+        # its companion stream and sets kernel attributes outside the capture
+        self.stream = torch.cuda.Stream(device=dev)
+        cur = torch.cuda.current_stream(dev)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            self._launch()
+        cur.wait_stream(self.stream)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            self._launch()
+
+    def _launch(self):
+        if self.forces:
+            energy_gradient(self.numbers, self.positions, self.charge, self.kp, q=self.q,
+                            r4r2=self.r4r2, ws=self.ws, energy=self.energy, grad=self.grad)
+        else:
+            forward(self.numbers, self.positions, self.charge, self.kp, q=self.q,
+                    r4r2=self.r4r2, ws=self.ws, energy=self.energy)
+
+    def replay(self):
+        self.graph.replay()
+        return (self.energy, self.grad) if self.forces else self.energy
+
+
 def molecule_status(ws: torch.Tensor, nbatch: int) -> np.ndarray:
     """Per-molecule status bits of the last call on ``ws`` (synchronises)."""
     out = np.zeros(nbatch, dtype=np.int32)
@@ -243,9 +295,17 @@ def molecule_status(ws: torch.Tensor, nbatch: int) -> np.ndarray:
     return out
 
 
-def profile_enable(device, on: bool, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT) -> None:
-    """Per-kernel HIP-event timing of calls through the context of ``device``."""
-    _lib.check(_lib.load().d4_profile_enable(context(device, ga, gc), int(bool(on))),
+def profile_enable(device, on: bool, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT,
+                   slots=None) -> None:
+    """Per-kernel HIP-event timing of calls through the context of ``device``:
+    every kernel, or only the kernel slots named in ``slots`` (names of
+    ``profile_read``)."""
+    code = int(bool(on))
+    if on and slots is not None:
+        code = _lib.D4_PROFILE_SLOTS
+        for name in slots:
+            code |= 1 << _lib.SLOTS.index(name)
+    _lib.check(_lib.load().d4_profile_enable(context(device, ga, gc), code),
                "d4_profile_enable")
 
 
