@@ -109,9 +109,11 @@ __global__ __launch_bounds__(256) void k_cn(DevTables t, Params p, Inputs in,
 // A double moved between lanes by one DPP control (both halves).
 template <int CTRL>
 __device__ __forceinline__ double dpp_double(double v) {
+  // every pattern used here is a full permutation of its lane group, so no
+  // lane keeps an old value (bound_ctrl; no copy of the source first)
   const int lo = __double2loint(v), hi = __double2hiint(v);
-  return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false));
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true),
+                          __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true));
 }
 
 constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]: lane ^ 1
@@ -1279,6 +1281,7 @@ __device__ __forceinline__ void ring_xor8(Atm8Ring& r, bool energy, bool grad) {
     r.F2 = xor_lanes8<D>(r.F2);
   }
 }
+
 
 constexpr double kAtmFar = 1.0e6;  // position of lanes and third atoms without an atom (bohr)
 // [z][x block | y block] pair records (t, p, kz, kc) of a chunk; the row
