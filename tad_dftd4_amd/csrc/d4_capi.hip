@@ -318,7 +318,8 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   const size_t ne = D4_MAX_Z, nr = size_t(D4_MAX_Z) * D4_MAX_REF;
   const size_t nrc6 = size_t(D4_MAX_Z) * D4_MAX_Z * D4_MAX_REF * D4_MAX_REF;
   const size_t nwf = size_t(D4_MAX_Z) * D4_MAX_Z;
-  const size_t ndbl = 10 * ne + 3 * nr + nrc6 + nwf;
+  const size_t npk = 4 * nwf;  // element-pair constants, filled on the device
+  const size_t ndbl = 10 * ne + 3 * nr + nrc6 + nwf + npk;
   const size_t nint = ne + nr;
   std::vector<double> hd;
   hd.reserve(ndbl);
@@ -331,6 +332,7 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
     hd.insert(hd.end(), tb->wfpair, tb->wfpair + nwf);
   else
     hd.insert(hd.end(), nwf, 0.0);
+  hd.insert(hd.end(), npk, 0.0);
   std::vector<int32_t> hi;
   hi.insert(hi.end(), tb->nref, tb->nref + ne);
   hi.insert(hi.end(), tb->ngw, tb->ngw + nr);
@@ -352,8 +354,8 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   if (e == hipSuccess)
     e = hipMemcpy((char*)mem + sizeof(double) * ndbl, hi.data(), sizeof(int32_t) * nint,
                   hipMemcpyHostToDevice);
-  (void)hipSetDevice(prev);
   if (e != hipSuccess) {
+    (void)hipSetDevice(prev);
     (void)hipFree(mem);
     return hip_error(e, "upload tables");
   }
@@ -378,9 +380,18 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   t.alpha0 = t.refq + nr;
   t.rc6 = t.alpha0 + nr;
   t.wfpair = t.rc6 + nrc6;
+  t.pairk = t.wfpair + nwf;
   const int32_t* ip = (const int32_t*)(d + ndbl);
   t.nref = ip;
   t.ngw = ip + ne;
+  e = d4::launch_pair_consts(nullptr, t, const_cast<double*>(t.pairk));
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  (void)hipSetDevice(prev);
+  if (e != hipSuccess) {
+    (void)hipFree(mem);
+    delete ctx;
+    return hip_error(e, "element-pair constants");
+  }
   *out = ctx;
   return D4_OK;
 }

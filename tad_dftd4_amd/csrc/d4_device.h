@@ -18,6 +18,8 @@ constexpr double kK4 = 4.10451;                       // defaults.py:44
 constexpr double kK5 = 19.08857;                      // defaults.py:47
 constexpr double kK6 = 2.0 * 11.28174 * 11.28174;     // defaults.py:50
 constexpr double kCnMax = 8.0;                        // defaults.py:32
+constexpr double kInvSqrtPi = 0.56418958354775628695;  // 1 / sqrt(pi)
+constexpr double kInvK6 = 1.0 / kK6;
 constexpr int kWaves = 4;                             // waves per 256-thread block
 constexpr int kTile = 16;                             // ATM pair tile edge
 constexpr int kZB = 32;                               // ATM third-atom rows per LDS round
@@ -59,21 +61,49 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
   return s;
 }
 
-// 1/2 (1 + erf(-k (r/rc - 1)))  (tad-mctc erf_count)
-__device__ __forceinline__ double erf_count(double r, double rc) {
-  return 0.5 * (1.0 + erf(-kKcn * (r / rc - 1.0)));
+// 1 / sqrt(x), x > 0: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt_nt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
+// 1 / x: hardware estimate and two Newton steps.
+__device__ __forceinline__ double rcp_nt(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/2 (1 + erf(-k (r/rc - 1)))  (tad-mctc erf_count), irc = 1 / rc
+__device__ __forceinline__ double erf_count(double r, double irc) {
+  return 0.5 * (1.0 + erf(-kKcn * fma(r, irc, -1.0)));
 }
 
 // d/dr of erf_count
-__device__ __forceinline__ double erf_count_dr(double r, double rc) {
-  const double x = r / rc - 1.0;
-  return -(kKcn / rc) * exp(-kKcn * kKcn * x * x) / sqrt(kPi);
+__device__ __forceinline__ double erf_count_dr(double r, double irc) {
+  const double x = fma(r, irc, -1.0);
+  return -(kKcn * kInvSqrtPi) * irc * exp(-kKcn * kKcn * x * x);
 }
 
 // Electronegativity factor of the D4 coordination number.
 __device__ __forceinline__ double en_factor(double eni, double enj) {
   const double d = fabs(eni - enj) + kK5;
-  return kK4 * exp(-(d * d) / kK6);
+  return kK4 * exp(-(d * d) * kInvK6);
+}
+
+// Element-pair constants (DevTables::pairk): 1 / (rcov_i + rcov_j), the
+// electronegativity factor and the EEQ gamma = 1 / sqrt(rad_i^2 + rad_j^2).
+struct PairConst {
+  double irc, enf, gam;
+};
+__device__ __forceinline__ PairConst pair_const(const DevTables& t, int zi, int zj) {
+  const double2* p = reinterpret_cast<const double2*>(t.pairk + (zi * kMaxZ + zj) * 4);
+  const double2 a = p[0], b = p[1];
+  return {a.x, a.y, b.x};
 }
 
 // Charge scaling function zeta and its derivative (model/base.py:308-365).
@@ -103,11 +133,14 @@ __device__ __forceinline__ double eeq_rhs(const DevTables& t, int z, double cnq)
   return -t.chi[z] + t.kcn[z] * sqrt(fmax(cnq, kEps));
 }
 
-__device__ __forceinline__ double eeq_offdiag(const DevTables& t, int zi, int zj,
-                                              double r) {
-  const double ri = t.rad[zi], rj = t.rad[zj];
-  const double gamma = 1.0 / sqrt(ri * ri + rj * rj);
-  return erf(r * gamma) / r;
+// Coulomb element erf(gamma r) / r from r^2 > 0.
+__device__ __forceinline__ double eeq_offdiag_r2(double gamma, double r2) {
+  const double ir = rsqrt_nt(r2);
+  return erf(r2 * ir * gamma) * ir;
+}
+
+__device__ __forceinline__ double eeq_offdiag(const DevTables& t, int zi, int zj, double r2) {
+  return eeq_offdiag_r2(t.pairk[(zi * kMaxZ + zj) * 4 + 2], r2);
 }
 
 __device__ __forceinline__ double eeq_diag(const DevTables& t, int z) {

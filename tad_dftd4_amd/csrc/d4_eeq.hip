@@ -395,8 +395,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
           double v = 0.0;
           if (ri != 0.0 && rj != 0.0) {
             const double dx = sx[i] - sx[j], dy = sy[i] - sy[j], dz = sz[i] - sz[j];
-            const double r = sqrt(dx * dx + dy * dy + dz * dz);
-            v = erf(r / sqrt(ri * ri + rj * rj)) / r;
+            v = eeq_offdiag_r2(rsqrt_nt(ri * ri + rj * rj), dx * dx + dy * dy + dz * dz);
           }
           L[tri(i) + j] = v;
         }
@@ -414,7 +413,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
           v = zi != 0 ? eeq_diag(t, zi) : 1.0;
         } else if (zi != 0 && zj != 0) {
           const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
-          v = eeq_offdiag(t, zi, zj, sqrt(dx * dx + dy * dy + dz * dz));
+          v = eeq_offdiag(t, zi, zj, dx * dx + dy * dy + dz * dz);
         } else {
           v = 0.0;
         }
@@ -478,7 +477,7 @@ __device__ void lu_assemble(const DevTables& t, const Inputs& in, const Work& w,
       } else if (zi != 0 && zj != 0) {
         const double dx = R[3 * i] - R[3 * j], dy = R[3 * i + 1] - R[3 * j + 1],
                      dz = R[3 * i + 2] - R[3 * j + 2];
-        v = eeq_offdiag(t, zi, zj, sqrt(dx * dx + dy * dy + dz * dz));
+        v = eeq_offdiag(t, zi, zj, dx * dx + dy * dy + dz * dz);
       } else {
         v = 0.0;
       }
@@ -666,7 +665,7 @@ __global__ __launch_bounds__(256) void k_tf_assemble(DevTables t, Params p, Inpu
         v = zi != 0 ? eeq_diag(t, zi) : 1.0;
       } else if (zi != 0 && zj != 0) {
         const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
-        v = eeq_offdiag(t, zi, zj, sqrt(dx * dx + dy * dy + dz * dz));
+        v = eeq_offdiag(t, zi, zj, dx * dx + dy * dy + dz * dz);
       } else {
         v = 0.0;
       }

@@ -36,6 +36,7 @@ struct DevTables {
   const double* alpha0;
   const double* rc6;
   const double* wfpair;  // D4S pairwise weighting factors [zi][zj]
+  const double* pairk;   // [zi][zj][4] element-pair constants (see pair_const)
   const int32_t* nref;
   const int32_t* ngw;
 };
@@ -323,6 +324,8 @@ hipError_t launch_properties(hipStream_t s, const DevTables& t,
                              const Inputs& in, const Work& w, double* c6_out,
                              double* alpha_out);
 hipError_t launch_copy(hipStream_t s, const double* src, double* dst, int64_t count);
+// Fill t.pairk (kMaxZ^2 x 4 doubles at `out`) from the per-element tables.
+hipError_t launch_pair_consts(hipStream_t s, const DevTables& t, double* out);
 // Sum of the parameter-gradient partials into out[7] = dL/d(s6, s8, s10, s9,
 // a1, a2, alp) (device pointer).
 hipError_t launch_pgrad_reduce(hipStream_t s, const Params& p, const Inputs& in,

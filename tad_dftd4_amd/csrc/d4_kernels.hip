@@ -79,15 +79,16 @@ __global__ __launch_bounds__(256) void k_cn(DevTables t, Params p, Inputs in,
     return;
   }
   const double xi = R[3 * i], yi = R[3 * i + 1], zzi = R[3 * i + 2];
-  const double rci = t.rcov[zi], eni = t.en[zi];
   double s4 = 0.0, se = 0.0;
   for (int j = lane; j < n; j += RL) {
     const int zj = (int)Z[j];
     if (j == i || zj == 0) continue;
     const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    const double c = erf_count(r, rci + t.rcov[zj]);
-    if (r <= p.cutcn) s4 += en_factor(eni, t.en[zj]) * c;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double r = r2 * rsqrt_nt(r2);
+    const PairConst pc = pair_const(t, zi, zj);
+    const double c = erf_count(r, pc.irc);
+    if (r <= p.cutcn) s4 += pc.enf * c;
     if (r <= p.cuteeq) se += c;
   }
   s4 = row_sum<RL>(s4);
@@ -2023,7 +2024,6 @@ __global__ __launch_bounds__(256) void k_chain(DevTables t, Params p, Inputs in,
   if (zi == 0) return;
   const bool eeq = in.q_in == nullptr;
   const double xi = R[3 * i], yi = R[3 * i + 1], zzi = R[3 * i + 2];
-  const double rci = t.rcov[zi], eni = t.en[zi], radi = t.rad[zi];
   const double dcni = w.dldcn[row], bi = w.beta[row], li = w.lam[row], qi = w.q[row];
   double fx = 0.0, fy = 0.0, fz = 0.0;
   for (int j = lane; j < n; j += RL) {
@@ -2031,19 +2031,22 @@ __global__ __launch_bounds__(256) void k_chain(DevTables t, Params p, Inputs in,
     if (j == i || zj == 0) continue;
     const int64_t rj = m * N + j;
     const double dx = xi - R[3 * j], dy = yi - R[3 * j + 1], dz = zzi - R[3 * j + 2];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    const double dc = erf_count_dr(r, rci + t.rcov[zj]);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double ir = rsqrt_nt(r2);
+    const double r = r2 * ir;
+    const PairConst pc = pair_const(t, zi, zj);
+    const double dc = erf_count_dr(r, pc.irc);
     double coef = 0.0;
-    if (r <= p.cutcn) coef += (dcni + w.dldcn[rj]) * en_factor(eni, t.en[zj]) * dc;
+    if (r <= p.cutcn) coef += (dcni + w.dldcn[rj]) * pc.enf * dc;
     if (eeq) {
       if (r <= p.cuteeq) coef += (bi + w.beta[rj]) * dc;
-      const double radj = t.rad[zj];
-      const double gm = 1.0 / sqrt(radi * radi + radj * radj);
-      const double dA = (2.0 * gm / sqrt(kPi)) * exp(-gm * gm * r * r) / r -
-                        erf(gm * r) / (r * r);
+      // d/dr erf(gamma r) / r
+      const double gm = pc.gam;
+      const double dA =
+          ir * fma(2.0 * kInvSqrtPi * gm, exp(-gm * gm * r2), -erf(gm * r) * ir);
       coef -= (li * w.q[rj] + w.lam[rj] * qi) * dA;
     }
-    coef /= r;
+    coef *= ir;
     fx += coef * dx;
     fy += coef * dy;
     fz += coef * dz;
@@ -2104,6 +2107,21 @@ __global__ __launch_bounds__(256) void k_properties(DevTables t, Inputs in, Work
     }
     alpha_out[row] = al;
   }
+}
+
+// Element-pair constants of the CN, chain and EEQ kernels (read through
+// pair_const): 1 / (rcov_i + rcov_j), the electronegativity factor and
+// 1 / sqrt(rad_i^2 + rad_j^2); zero where an element has no radius.
+__global__ __launch_bounds__(256) void k_pair_consts(DevTables t, double* out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= kMaxZ * kMaxZ) return;
+  const int zi = k / kMaxZ, zj = k - zi * kMaxZ;
+  const double rc = t.rcov[zi] + t.rcov[zj];
+  const double rr = t.rad[zi] * t.rad[zi] + t.rad[zj] * t.rad[zj];
+  out[4 * k] = rc > 0.0 ? 1.0 / rc : 0.0;
+  out[4 * k + 1] = en_factor(t.en[zi], t.en[zj]);
+  out[4 * k + 2] = rr > 0.0 ? 1.0 / sqrt(rr) : 0.0;
+  out[4 * k + 3] = 0.0;
 }
 
 __global__ __launch_bounds__(256) void k_copy(const double* src, double* dst,
@@ -2383,6 +2401,11 @@ hipError_t launch_pgrad_reduce(hipStream_t s, const Params& p, const Inputs& in,
                                       : nb * (nb + 1) / 2;
   hipLaunchKernelGGL(k_pgrad_reduce, dim3(1), dim3(256), 0, s, in, w, mode, tile8 ? 1 : 0,
                      per_mol, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_consts(hipStream_t s, const DevTables& t, double* out) {
+  hipLaunchKernelGGL(k_pair_consts, dim3((kMaxZ * kMaxZ + 255) / 256), dim3(256), 0, s, t, out);
   return hipGetLastError();
 }
 
