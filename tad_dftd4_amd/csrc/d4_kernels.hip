@@ -328,7 +328,8 @@ __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, c
   constexpr bool ATMF = PART != kPartE2, E2 = PART != kPartAtm;
   const double dx = A.x - xj, dy = A.y - yj, dz = A.zc - zzj;
   const double r2 = dx * dx + dy * dy + dz * dz;
-  const double r = sqrt(r2);
+  const double ir = rsqrt_nt(r2);
+  const double r = r2 * ir;
   double gwj[kMaxRef], gw0j[kMaxRef];
 #pragma unroll
   for (int b = 0; b < kMaxRef; ++b) {
@@ -384,21 +385,22 @@ __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, c
   const double r0 = p.a1 * sqrt(qq) + p.a2;
   tf = pf = pk = 0.0;
   if (ATMF) {
-    // ATM pair factors (threebody.py:106-145, 241-256, 311-321)
-    tf = pow(r0 / r, p.alp / 3.0);
-    pf = sqrt(fabs(c60)) / (r2 * r);
+    // ATM pair factors (threebody.py:106-145, 241-256, 311-321); r0 > 0, so
+    // (r0 / r)^(alp / 3) = exp(alp / 3 log(r0 / r))
+    tf = exp((p.alp * (1.0 / 3.0)) * log(r0 * ir));
+    pf = sqrt(fabs(c60)) * (ir * ir * ir);
     if (GRAD) pk = c60 != 0.0 ? 0.5 * dc60cn / c60 : 0.0;
   }
   if (!E2) return;
   const double r6 = r2 * r2 * r2, r8 = r6 * r2;
   const double r06 = r0 * r0 * r0 * r0 * r0 * r0, r08 = r06 * r0 * r0;
-  const double t6 = 1.0 / (r6 + r06);
-  const double t8 = 1.0 / (r8 + r08);
+  const double t6 = rcp_nt(r6 + r06);  // r0^6 > 0
+  const double t8 = rcp_nt(r8 + r08);
   double de_dc6 = p.s6 * t6 + p.s8 * qq * t8;
   double t10 = 0.0, r10 = 0.0;
   if (p.s10on) {
     r10 = r8 * r2;
-    t10 = 1.0 / (r10 + r08 * r0 * r0);
+    t10 = rcp_nt(r10 + r08 * r0 * r0);
     de_dc6 += p.s10 * qq * qq * (49.0 / 40.0) * t10;
   }
   de_dc6 *= -0.5;
@@ -422,10 +424,10 @@ __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, c
     acc.cn += wij * de_dc6 * dc6cn;
     acc.q += wij * de_dc6 * dc6q;
     // d/dr of 1/(r^n + r0^n) = -n r^(n-1) t^2
-    double dedr = p.s6 * (-6.0 * r6 / r) * t6 * t6 + p.s8 * qq * (-8.0 * r8 / r) * t8 * t8;
-    if (p.s10on) dedr += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r10 / r) * t10 * t10;
+    double dedr = p.s6 * (-6.0 * r6 * ir) * t6 * t6 + p.s8 * qq * (-8.0 * r8 * ir) * t8 * t8;
+    if (p.s10on) dedr += p.s10 * qq * qq * (49.0 / 40.0) * (-10.0 * r10 * ir) * t10 * t10;
     dedr *= -0.5 * c6;
-    const double coef = wij * dedr / r;
+    const double coef = wij * dedr * ir;
     acc.fx += coef * dx;
     acc.fy += coef * dy;
     acc.fz += coef * dz;
@@ -874,14 +876,6 @@ __device__ __forceinline__ double rcp_newton1(double d) {
   return fma(r, fma(-d, r, 1.0), r);
 }
 
-// 1 / d by the hardware estimate and two Newton steps.
-__device__ __forceinline__ double fast_rcp(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
-}
 
 // Pair data of (z, c) staged for the one-pass kernel, c an atom of the tile.
 struct ZPair {
@@ -1073,7 +1067,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
           const double bc = bb * cc;
           const double abc = a * bc;
           const double opT = fma(6.0, T, 1.0);
-          const double q0 = fast_rcp(abc * opT);
+          const double q0 = rcp_nt(abc * opT);
           const double f = abc * q0;     // 1 / (1 + 6 T)
           const double iabc = opT * q0;  // 1 / (a b c)
           const double G2 = 0.375 * iabc;
@@ -1134,7 +1128,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
               g_s9 = fma(W6, P0 * f * g, g_s9);
               g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, g_alp);
               const double sxz = sq3 * rx4 * s_rt[zl], syz = sq3 * ry4 * s_rt[zl];
-              const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
+              const double bxz = rcp_nt(fma(a1p, sxz, a2p)), byz = rcp_nt(fma(a1p, syz, a2p));
               g_a1 = fma(TdT * alp / 3.0, Axy + sxz * bxz + syz * byz, g_a1);
               g_a2 = fma(TdT * alp / 3.0, Bxy + bxz + byz, g_a2);
             }
@@ -1532,7 +1526,7 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
         u.g_s9 = fma(W6, P0 * f * g, u.g_s9);
         u.g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, u.g_alp);
         const double sxz = sq3 * q.rx4 * s_rt[zl], syz = sq3 * q.ry4 * s_rt[zl];
-        const double bxz = fast_rcp(fma(a1p, sxz, a2p)), byz = fast_rcp(fma(a1p, syz, a2p));
+        const double bxz = rcp_nt(fma(a1p, sxz, a2p)), byz = rcp_nt(fma(a1p, syz, a2p));
         u.g_a1 = fma(TdT * alp / 3.0, q.Axy + sxz * bxz + syz * byz, u.g_a1);
         u.g_a2 = fma(TdT * alp / 3.0, q.Bxy + bxz + byz, u.g_a2);
       }
