@@ -116,16 +116,20 @@ __device__ __forceinline__ double zeta_fn(double ga, double gam, double qref,
   return exp(ga);
 }
 
-__device__ __forceinline__ double dzeta_fn(double ga, double gam, double qref,
-                                           double qmod) {
+// zeta and its derivative together (the exponentials shared).
+__device__ __forceinline__ void zeta_dzeta(double ga, double gam, double qref, double qmod,
+                                           double& z, double& dz) {
   if (qmod > 0.0) {
     const double den = qmod - kEps;
     const double scale = exp(gam * (1.0 - qref / den));
-    const double z = exp(ga * (1.0 - scale));
-    return -ga * gam * scale * z * qref / (den * den);
+    z = exp(ga * (1.0 - scale));
+    dz = -ga * gam * scale * z * qref / (den * den);
+  } else {
+    z = exp(ga);
+    dz = 0.0;
   }
-  return 0.0;
 }
+
 
 __host__ __device__ __forceinline__ int tri(int i) { return (i * (i + 1)) >> 1; }
 

@@ -149,9 +149,10 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
   const double zeff = t.zeff[z];
   const double gam = t.gam[z] * p.gc;
   const double qref = used ? t.refq[z * kMaxRef + a] + zeff : 0.0;
-  const double zq = used ? zeta_fn(p.ga, gam, qref, q + zeff) : 0.0;
-  const double z0 = used ? zeta_fn(p.ga, gam, qref, zeff) : 0.0;
-  const double dzq = used ? dzeta_fn(p.ga, gam, qref, q + zeff) : 0.0;
+  // charge scaling of the part in hand: zeta(q) and dzeta/dq, zeta(0)
+  double zq = 0.0, dzq = 0.0, z0 = 0.0;
+  if (used && two) zeta_dzeta(p.ga, gam, qref, q + zeff, zq, dzq);
+  if (used && atm) z0 = zeta_fn(p.ga, gam, qref, zeff);
   const int64_t o = row * kRefStride + a;
   if (p.d4s) {
     // D4S: the Gaussian part depends on the partner atom and is formed in
@@ -170,15 +171,16 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
   }
   double e = 0.0, de = 0.0;
   if (used) {
-    // exp(-wf k d^2) evaluated as exp(-d^2)^(wf k), the reference's order
-    // of operations (model/d4.py:174-188): identical rounding where the
-    // Gaussians approach underflow
+    // the reference forms exp(-d^2)^(wf k) (model/d4.py:174-188); one
+    // exponential of -wf k d^2 is the same value to ~1e-13 relative, and
+    // underflows where it does (both round one exponent of nearly the same
+    // argument)
     const double d = cn - t.refcn[z * kMaxRef + a];
-    const double tmp = exp(-(d * d));
+    const double d2 = d * d;
     const int ng = t.ngw[z * kMaxRef + a];
     for (int k = 1; k <= ng; ++k) {
       const double kw = k * p.wf;
-      const double ek = pow(tmp, kw);
+      const double ek = exp(-kw * d2);
       e += ek;
       de -= 2.0 * kw * d * ek;
     }
