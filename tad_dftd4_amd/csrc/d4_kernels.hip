@@ -15,6 +15,9 @@
 
 #include "d4_device.h"
 
+#define D4_PRAGMA(x) _Pragma(#x)
+#define D4_UNROLL(n) D4_PRAGMA(unroll n)
+
 namespace d4 {
 namespace {
 
@@ -474,9 +477,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int n = w.natom[m];
   const int64_t* Z = in.numbers + m * N;
   const double* R = in.positions + m * N * 3;
-  double* ptr = w.pt + row * N;
-  double* ppr = w.pp + row * N;
-  double* pkr = w.pk + row * N;
+  double* __restrict__ ptr = w.pt + row * N;
+  double* __restrict__ ppr = w.pp + row * N;
+  double* __restrict__ pkr = w.pk + row * N;
+  const double* __restrict__ gw = w.gw;
+  const double* __restrict__ gw0 = w.gw0;
   const int zi = i < n ? (int)Z[i] : 0;
   if (zi == 0) {
     if (ATMF) {
@@ -512,8 +517,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const double gj = (GRAD && E2) ? (in.gE ? in.gE[rj] : 1.0) : 0.0;
     double tf, pf, pk;
     pair_eval<GRAD, PG, S, PART>(t, p, A, zj, R[3 * j], R[3 * j + 1], R[3 * j + 2], r4j, cnj,
-                                 gj, w.gw + rj * kRefStride, w.gw0 + rj * kRefStride, acc, tf,
-                                 pf, pk);
+                                 gj, gw + rj * kRefStride, gw0 + rj * kRefStride, acc, tf, pf,
+                                 pk);
     if (ATMF) {
       ptr[j] = tf;
       ppr[j] = pf;
@@ -893,8 +898,6 @@ struct ZPair {
 #ifndef D4_ATM1_WAVES
 #define D4_ATM1_WAVES 1
 #endif
-#define D4_PRAGMA(x) _Pragma(#x)
-#define D4_UNROLL(n) D4_PRAGMA(unroll n)
 
 template <bool ENERGY, bool GRAD, bool PG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAVES))) void k_atm1(
