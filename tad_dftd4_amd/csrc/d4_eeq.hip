@@ -523,13 +523,9 @@ __device__ void lu_solve(const double* A, int ld, const int32_t* piv, int n1,
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_eeq_lu(DevTables t, Params p, Inputs in,
-                                                Work w) {
-  __shared__ double s_val[256];
-  __shared__ int s_idx[256];
-  const int64_t m = blockIdx.x;
-  if (in.q_in != nullptr) return;
-  if (!(w.status[m] & kStatusLU)) return;
+// Pivoted LU of the bordered system of molecule m (work-group wide).
+__device__ void eeq_lu_molecule(const DevTables& t, const Inputs& in, const Work& w, int64_t m,
+                                double* s_val, int* s_idx) {
   const int n = w.natom[m];
   const int n1 = n + 1;
   const int ld = (int)w.N + 1;
@@ -599,6 +595,19 @@ __global__ __launch_bounds__(256) void k_eeq_lu(DevTables t, Params p, Inputs in
   lu_solve(A, ld, piv, n1, vec);
   for (int i = tid; i < n; i += 256)
     w.q[m * w.N + i] = in.numbers[m * w.N + i] != 0 ? vec[i] : 0.0;
+}
+
+// The molecules whose Cholesky pivot was <= 0 (or all, when forced): a few
+// work-groups walk the batch, so the launch costs little when none needs it.
+__global__ __launch_bounds__(256) void k_eeq_lu(DevTables t, Params p, Inputs in, Work w) {
+  __shared__ double s_val[256];
+  __shared__ int s_idx[256];
+  if (in.q_in != nullptr) return;
+  for (int64_t m = blockIdx.x; m < w.B; m += gridDim.x) {
+    if (!(w.status[m] & kStatusLU)) continue;  // uniform over the work-group
+    eeq_lu_molecule(t, in, w, m, s_val, s_idx);
+    __syncthreads();
+  }
 }
 
 // ==========================================================================
@@ -1095,6 +1104,8 @@ hipError_t launch_eeq_tiled(hipStream_t s, const DevTables& t, const Params& p,
   return hipGetLastError();
 }
 
+constexpr unsigned kLuGroups = 64;  // work-groups of the LU fallback launch
+
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
                       const Inputs& in, const Work& w, bool /*keep_factor*/) {
   static size_t configured[2] = {0, 0};
@@ -1111,7 +1122,8 @@ hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
       e = launch_lds(k_eeq_chol<kEeqFactorLds>, configured[0], grid, shmem, s, t, p, in, w);
   }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_eeq_lu, dim3(grid), dim3(256), 0, s, t, p, in, w);
+  hipLaunchKernelGGL(k_eeq_lu, dim3(grid < kLuGroups ? grid : kLuGroups), dim3(256), 0, s, t, p,
+                     in, w);
   return hipGetLastError();
 }
 

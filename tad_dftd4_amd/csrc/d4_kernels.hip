@@ -1970,28 +1970,35 @@ __global__ __launch_bounds__(1024) void k_atm8s_rows(Inputs in, Work w, double* 
 }
 
 // Atom sums of the one-pass three-body kernel over its splits: energies
-// (with the two-body part), gradient and dL/dCN.
+// (with the two-body part), gradient and dL/dCN. Eight lanes per atom: lane
+// l adds splits l, l + 8, ..., then the eight partial sums are added in a
+// fixed order (DPP), so the sum depends on the molecule's atom count alone.
 template <bool ENERGY, bool GRAD>
 __global__ __launch_bounds__(256) void k_atm1_rows(Inputs in, Work w, int tile8,
                                                    double* energy_out, double* grad_out) {
-  const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (row >= w.B * w.N) return;
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t row = tid >> 3;
+  const int l = (int)(tid & 7);
+  if (row >= w.B * w.N) return;  // uniform over each group of 8 lanes
   const int64_t N = w.N;
   const int64_t m = row / N;
   const int i = (int)(row - m * N);
   const int n = w.natom[m];
   const int zi = i < n ? (int)in.numbers[row] : 0;
   if (zi == 0) {
-    if (ENERGY) energy_out[row] = 0.0;
+    if (ENERGY && l == 0) energy_out[row] = 0.0;
     return;
   }
   const int S = one_pass_splits(n, tile8 != 0);
   const int64_t stride = (int64_t)one_pass_splits((int)N, tile8 != 0) * kAtm1Acc * N;
   const double* part = w.apart + m * stride + i;
   double acc[kAtm1Acc] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int s = 0; s < S; ++s)
+  for (int s = l; s < S; s += 8)
 #pragma unroll
     for (int c = 0; c < kAtm1Acc; ++c) acc[c] += part[(s * kAtm1Acc + c) * N];
+#pragma unroll
+  for (int c = 0; c < kAtm1Acc; ++c) acc[c] = sum_lanes8(acc[c]);
+  if (l != 0) return;
   const double sixth = 1.0 / 6.0;
   if (ENERGY) energy_out[row] = w.e2[row] + acc[0] * sixth;
   if (GRAD) {
@@ -2338,7 +2345,7 @@ hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, con
                            bool energy, bool grad, double* energy_out,
                            double* grad_out) {
   if (one_pass(p, w)) {
-    const unsigned grid = (unsigned)((w.B * w.N + 255) / 256);
+    const unsigned grid = (unsigned)((w.B * w.N * 8 + 255) / 256);
     const int tile8 = one_pass_tile8(p, grad) ? 1 : 0;
     if (energy && grad)
       hipLaunchKernelGGL((k_atm1_rows<true, true>), dim3(grid), dim3(256), 0, s, in, w, tile8,
