@@ -915,8 +915,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
   __shared__ double s_red[kTile][kTile][NA];
   __shared__ double s_acc[kAtm1MaxN][NA];
 
-  const int64_t G = gridDim.x, b = blockIdx.x;
-  const int64_t logical = (b % 8) * (G / 8) + b / 8;  // consecutive blocks on one XCD
+  const int64_t b = blockIdx.x;
+  // block b runs on XCD b % 8: consecutive splits of a molecule go to
+  // different XCDs, which evens out the XCDs' shares of a batch of mixed sizes
+  const int64_t logical = b;
   if (logical >= nblocks) return;
   const int64_t m = logical / S;
   const int s = (int)(logical - m * S);
