@@ -1258,7 +1258,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
 // step k, and the running sums for z move to the lane that handles z at the
 // next step: lane L takes the sums of lane L ^ (k ^ (k + 1)), a swap inside
 // each quad (xor 1 or 3) or a mirror of the 8-lane half row (xor 7), single
-// DPP moves. After 8 steps lane tx holds its row's sums for z0 + (tx ^ 7).
+// DPP moves; one more xor 7 after the last step (so the moves repeat every
+// four steps) leaves lane tx with its row's sums for z0 + tx.
 // Every sum is added in a fixed order (no atomics).
 // --------------------------------------------------------------------------
 
@@ -1428,14 +1429,17 @@ __device__ __forceinline__ void atm8_stage(const Atm8Chunk& c, double* s_zc, dou
 // lane (tx, ty). FULL: every pair within the cutoff (weights sx = sy = sz =
 // 2; the factor 2 of the energies is applied by the caller); FIRST: the chunk
 // of y's own block, where third atoms up to y carry no weight. On return lane
-// tx of every row holds that row's ring sums for z0 + (tx ^ 7).
+// tx of every row holds that row's ring sums for z0 + tx.
 template <bool ENERGY, bool GRAD, bool PG, bool FULL, bool FIRST>
 __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, Atm8Sums& u,
                                            Atm8Ring& rr, const double* s_zc, const double* s_r,
                                            const double* s_rt, int tx, int ty, int z0, int y) {
   constexpr int T8 = kAtm8;
   const double c3sq = p.cut3sq, alp = p.alp, a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
-#pragma unroll 2
+  // four steps per iteration (two with parameter gradients, whose extra
+  // sums would not fit the registers), so every ring move below is static
+  constexpr int kUnroll = PG ? 2 : 4;
+  D4_UNROLL(kUnroll)
   for (int k = 0; k < T8; ++k) {
     const int zl = tx ^ k;
     const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
@@ -1539,13 +1543,13 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
       }
     }
     // hand the sums for z to the lane that takes z at the next step:
-    // k ^ (k + 1) is 1, 3, 1, 7, 1, 3, 1 (only the last step of each group
-    // of four depends on the group)
+    // k ^ (k + 1) is 1, 3, 1, 7, 1, 3, 1, and one more xor 7 after the last
+    // step (the same four moves in every group of four steps)
     if ((k & 1) == 0)
       ring_xor8<1>(rr, ENERGY, GRAD);
     else if ((k & 3) == 1)
       ring_xor8<3>(rr, ENERGY, GRAD);
-    else if (k == 3)
+    else
       ring_xor8<7>(rr, ENERGY, GRAD);
   }
 }
@@ -1667,13 +1671,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
                                                      y);
       }
       if (full) rr.E *= 2.0;
-      // lane tx of every row now holds that row's sums for z0 + (tx ^ 7) (the
+      // lane tx of every row now holds that row's sums for z0 + tx (the
       // gradient parts at half weight); add the 8 rows in order
       atm8_stage_sums<NA>(s_red, lane, rr.E, rr.F0, rr.F1, rr.F2, rr.D);
       __syncthreads();
       if (lane < T8 * NA) {
-        const int l = lane & (T8 - 1), c = lane >> 3;  // column l holds z0 + (l ^ 7)
-        const int z = z0 + (l ^ 7);
+        const int l = lane & (T8 - 1), c = lane >> 3;  // column l holds z0 + l
+        const int z = z0 + l;
         if ((c == 0 ? ENERGY : GRAD) && z < n) {
           double sum = 0.0;
           for (int r = 0; r < T8; ++r) sum += s_red[c][r * T8 + l];
@@ -1862,9 +1866,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const int gzn = next_z();
         if (gzn < G)  // in flight during this chunk
           atm8_fetch<GRAD, PG>(ch, in, w, m, n, gzn * T8, gx * T8, gy * T8, lane);
-        // lane c * 8 + l adds the sums of component c for z = 8 gz + (l ^ 7)
+        // lane c * 8 + l adds the sums of component c for z = 8 gz + l
         double* wz = slot_of(gz);
-        const int widx = (lane & ~(T8 - 1)) | ((lane & (T8 - 1)) ^ 7);
+        const int widx = lane;
         const double old = lane < kWin ? wz[widx] : 0.0;
         Atm8Ring rr = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (gz == gy)
