@@ -84,3 +84,27 @@ def test_large_system_replays_match_eager():
         e, g = cap.replay()
         torch.cuda.synchronize()
         assert torch.equal(e, e0) and torch.equal(g, g0), k
+
+
+def test_profile_slot_mask():
+    """Kernel timing restricted to one slot brackets only that kernel's
+    launches; with every slot enabled all of them are timed."""
+    _say("profiling: slot mask vs all slots")
+    numbers, positions, charge = _batch(403, (12, 30))
+    kp = _kp()
+    dev = positions.device
+    runtime.energy_gradient(numbers, positions, charge, kp)
+    torch.cuda.synchronize()
+    runtime.profile_read(dev)  # clear
+    runtime.profile_enable(dev, True, slots=("atm",))
+    runtime.energy_gradient(numbers, positions, charge, kp)
+    runtime.profile_enable(dev, False)
+    only = runtime.profile_read(dev)
+    assert only["atm"][1] == 1 and only["atm"][0] > 0.0
+    assert all(v[1] == 0 for k, v in only.items() if k != "atm")
+    runtime.profile_enable(dev, True)
+    runtime.energy_gradient(numbers, positions, charge, kp)
+    runtime.profile_enable(dev, False)
+    every = runtime.profile_read(dev)
+    for k in ("cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain"):
+        assert every[k][1] >= 1, k
