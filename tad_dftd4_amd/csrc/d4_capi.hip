@@ -240,6 +240,7 @@ hipError_t join_aux(Call& c) {
 int run_cn(d4_context* ctx, Call& c) {
   D4_RUN(D4_SLOT_PREP, enter_sorted(c, false), "spatial order");
   D4_RUN(D4_SLOT_PREP, d4::launch_prep(c.s, c.in, c.w), "prep");
+  if (!c.p.d4s) D4_RUN(D4_SLOT_PREP, d4::launch_zslots(c.s, c.in, c.w), "element slots");
   D4_RUN(D4_SLOT_CN, d4::launch_cn(c.s, ctx->t, c.p, c.in, c.w), "coordination numbers");
   return D4_OK;
 }
@@ -279,9 +280,12 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
     return D4_OK;
   }
   D4_TRY(fork_aux(ctx, c), "fork companion stream");
-  if (eeq)
+  if (eeq) {
     D4_RUN_AUX(D4_SLOT_WEIGHTS, d4::launch_weights(c.aux.s, ctx->t, c.p, c.w, c.in, d4::kPairsAtm),
                "reference weights (q = 0)");
+    D4_RUN_AUX(D4_SLOT_WEIGHTS, d4::launch_pairvec(c.aux.s, ctx->t, c.w, c.in, d4::kPairsAtm),
+               "C6 contraction vectors (q = 0)");
+  }
   D4_RUN_AUX(D4_SLOT_PAIRS,
              d4::launch_pairs(c.aux.s, ctx->t, c.p, c.in, c.w, grad, nullptr, d4::kPairsAtm),
              "three-body pair factors");
@@ -290,6 +294,8 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
     D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
     D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsTwoBody),
            "reference weights");
+    D4_RUN(D4_SLOT_WEIGHTS, d4::launch_pairvec(c.s, ctx->t, c.w, c.in, d4::kPairsTwoBody),
+           "C6 contraction vectors");
   }
   D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, grad, g_out, d4::kPairsTwoBody),
          "two-body term");

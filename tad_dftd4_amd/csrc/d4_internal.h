@@ -13,6 +13,9 @@ constexpr int kMaxZ = D4_MAX_Z;
 constexpr int kMaxRef = D4_MAX_REF;
 constexpr int kRefStride = 8;  // per-atom weight vectors are padded to 8
 constexpr int kRc6Block = kMaxRef * kMaxRef;
+// Distinct elements of a molecule given C6 contraction vectors (more: the pair
+// passes contract the reference C6 blocks per pair).
+constexpr int kZSlots = 32;
 
 // Molecule status bits (d4_molecule_status).
 constexpr int kStatusLU = 1;
@@ -101,6 +104,11 @@ struct Work {
   int64_t* wix;     // [B*(2*NG8+3)] slot tables of the screened one-pass kernel
   double* apart;    // [B*S*5*N] per-split atom sums of the one-pass three-body kernel
   double* rt4;      // [B*N] sqrt(r4r2) per atom (parameter gradients)
+  int32_t* zslot;   // [B*kMaxZ] element -> slot in the molecule's element list (-1: none)
+  int32_t* nzs;     // [B] distinct elements of the molecule, -1 if more than kZSlots
+  int32_t* zlist;   // [B*kZSlots] slot -> element
+  double* cv;       // [B*N*kZSlots*8] sum_b RC6[Z_e][Z_j][a][b] gw_j[b] per atom j, slot e
+  double* cv0;      // the same with the weights at q = 0
   double* pg2;      // [B*N*5] two-body parameter-gradient partials per atom row
   double* pg3;      // [B*P*4] three-body parameter-gradient partials per work-group
   int64_t B, N;
@@ -278,6 +286,11 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
                                     size_t(N)
                               : 0);
   v.rt4 = (double*)take(8 * bn);
+  v.zslot = (int32_t*)take(4 * size_t(B) * kMaxZ);
+  v.nzs = (int32_t*)take(4 * size_t(B));
+  v.zlist = (int32_t*)take(4 * size_t(B) * kZSlots);
+  v.cv = (double*)take(8 * bn * kZSlots * 8);
+  v.cv0 = (double*)take(8 * bn * kZSlots * 8);
   v.pg2 = (double*)take(8 * bn * 5);
   v.pg3 = (double*)take(32 * size_t(B) * size_t(pgrad_partials(N)));
   if (w) *w = v;
@@ -308,6 +321,11 @@ constexpr int kPairsAtm = 1;
 constexpr int kPairsTwoBody = 2;
 hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
                           const Work& w, const Inputs& in, int part);
+// Element slots of every molecule (after launch_prep), and the C6 contraction
+// vectors of one part (after its weights; D4 model only).
+hipError_t launch_zslots(hipStream_t s, const Inputs& in, const Work& w);
+hipError_t launch_pairvec(hipStream_t s, const DevTables& t, const Work& w, const Inputs& in,
+                          int part);
 hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, bool grad,
                         double* grad_out, int part);

@@ -284,6 +284,69 @@ constexpr int kPartBoth = 0;
 constexpr int kPartAtm = 1;  // t, p, d log sqrt|C6(0)| / d CN (and sqrt r4r2 for PG)
 constexpr int kPartE2 = 2;   // two-body energy, dL/dCN, dL/dq, forces, PG partials
 
+// Element slots of a molecule (one wave per molecule): its distinct elements
+// in ascending order; a molecule with more than kZSlots elements gets none
+// (nzs = -1) and its pair passes contract the reference C6 blocks per pair.
+__global__ __launch_bounds__(64) void k_zslots(Inputs in, Work w) {
+  __shared__ int present[kMaxZ];
+  const int64_t m = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int n = w.natom[m];
+  const int64_t* Z = in.numbers + m * w.N;
+  for (int z = lane; z < kMaxZ; z += 64) present[z] = 0;
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) {
+    const int64_t z = Z[i];
+    if (z > 0 && z < kMaxZ) present[z] = 1;
+  }
+  __syncthreads();
+  if (lane != 0) return;
+  int count = 0;
+  for (int z = 1; z < kMaxZ; ++z) count += present[z];
+  const bool ok = count <= kZSlots;
+  int s = 0;
+  for (int z = 0; z < kMaxZ; ++z) {
+    int slot = -1;
+    if (ok && z > 0 && present[z]) {
+      w.zlist[m * kZSlots + s] = z;
+      slot = s++;
+    }
+    w.zslot[m * kMaxZ + z] = slot;
+  }
+  w.nzs[m] = ok ? count : -1;
+}
+
+// C6 contraction vectors: for atom j and each element slot e of its molecule,
+// v[a] = sum_b RC6[Z_e][Z_j][a][b] gw_j[b] (a < 7; entry 7 is zero), with the
+// weights at q (two-body part) or q = 0 (three-body part). Eight lanes per
+// atom, lane a walks the slots.
+__global__ __launch_bounds__(256) void k_pairvec(DevTables t, Work w, Inputs in, int part) {
+  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t row = tid >> 3;
+  const int a = (int)(tid & 7);
+  if (row >= w.B * w.N) return;
+  const int64_t m = row / w.N;
+  const int j = (int)(row - m * w.N);
+  const int nz = w.nzs[m];
+  const int zj = j < w.natom[m] ? (int)in.numbers[row] : 0;
+  if (zj == 0 || nz <= 0) return;
+  const double* gwj = (part == kPairsAtm ? w.gw0 : w.gw) + row * kRefStride;
+  double g[kMaxRef];
+#pragma unroll
+  for (int b = 0; b < kMaxRef; ++b) g[b] = gwj[b];
+  double* out = (part == kPairsAtm ? w.cv0 : w.cv) + row * kZSlots * 8 + a;
+  for (int e = 0; e < nz; ++e) {
+    double v = 0.0;
+    if (a < kMaxRef) {
+      const double* blk = t.rc6 + ((size_t)w.zlist[m * kZSlots + e] * kMaxZ + zj) * kRc6Block +
+                          a * kMaxRef;
+#pragma unroll
+      for (int b = 0; b < kMaxRef; ++b) v += blk[b] * g[b];
+    }
+    out[e * 8] = v;
+  }
+}
+
 // Atom i of a pair (the atom whose sums are accumulated).
 struct PairAtom {
   int z, nr;
@@ -328,18 +391,21 @@ template <bool GRAD, bool PG, bool S, int PART>
 __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, const PairAtom& A,
                                           int zj, double xj, double yj, double zzj, double r4j,
                                           double cnj, double gj, const double* gwjp,
-                                          const double* gw0jp, PairSums& acc, double& tf,
-                                          double& pf, double& pk) {
+                                          const double* gw0jp, const double* vj,
+                                          PairSums& acc, double& tf, double& pf, double& pk) {
   constexpr bool ATMF = PART != kPartE2, E2 = PART != kPartAtm;
   const double dx = A.x - xj, dy = A.y - yj, dz = A.zc - zzj;
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double ir = rsqrt_nt(r2);
   const double r = r2 * ir;
+  // D4 split passes of molecules with element slots: the partner's weights
+  // come precontracted with the reference C6 block of this atom's element
+  const bool pv = !S && PART != kPartBoth && vj != nullptr;
   double gwj[kMaxRef], gw0j[kMaxRef];
 #pragma unroll
   for (int b = 0; b < kMaxRef; ++b) {
-    gwj[b] = E2 ? gwjp[b] : 0.0;
-    gw0j[b] = ATMF ? gw0jp[b] : 0.0;
+    gwj[b] = (E2 && !pv) ? gwjp[b] : 0.0;
+    gw0j[b] = (ATMF && !pv) ? gw0jp[b] : 0.0;
   }
   // D4S: Gaussian weights of both atoms for this pair; atom n's weights for
   // partner m use wf(Z_n, Z_m) (model/d4s.py:196). The weights of atom i
@@ -357,9 +423,26 @@ __device__ __forceinline__ void pair_eval(const DevTables& t, const Params& p, c
   }
   const double* blk = t.rc6 + ((size_t)A.z * kMaxZ + zj) * kRc6Block;
   double c6 = 0.0, c60 = 0.0, dc6cn = 0.0, dc6q = 0.0, dc60cn = 0.0;
+  if (pv) {
+    // v[a] = sum_b RC6[Z_i][Z_j][a][b] gw_j[b], summed as below; the weights
+    // of reference systems a >= nref are zero, so the sums match bit for bit
+    const double2* v2 = reinterpret_cast<const double2*>(vj);
+    const double2 v01 = v2[0], v23 = v2[1], v45 = v2[2], v67 = v2[3];
+    const double v[kMaxRef] = {v01.x, v01.y, v23.x, v23.y, v45.x, v45.y, v67.x};
+#pragma unroll
+    for (int a = 0; a < kMaxRef; ++a) {
+      if (E2) c6 += A.gw[a] * v[a];
+      if (ATMF) c60 += A.gw0[a] * v[a];
+      if (GRAD && E2) {
+        dc6cn += A.dgw[a] * v[a];
+        dc6q += A.dgwq[a] * v[a];
+      }
+      if (GRAD && ATMF) dc60cn += A.dgw0[a] * v[a];
+    }
+  }
 #pragma unroll
   for (int a = 0; a < kMaxRef; ++a) {
-    if (a >= A.nr) continue;
+    if (pv || a >= A.nr) continue;
     double va = 0.0, v0a = 0.0;
 #pragma unroll
     for (int b = 0; b < kMaxRef; ++b) {
@@ -497,6 +580,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   PairAtom A;
   load_pair_atom<GRAD, S, PART>(t, w, in, row, zi, R, i, A);
   PairSums acc;
+  const int slot = (!S && PART != kPartBoth) ? w.zslot[m * kMaxZ + zi] : -1;
+  const double* cvp = PART == kPartAtm ? w.cv0 : w.cv;
   // the partner's Z is loaded one iteration ahead: the table reads of a pair
   // (reference C6 block, r4r2) depend on it
   int znext = lane < n ? (int)Z[lane] : 0;
@@ -516,9 +601,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const double cnj = S ? w.cn[rj] : 0.0;
     const double gj = (GRAD && E2) ? (in.gE ? in.gE[rj] : 1.0) : 0.0;
     double tf, pf, pk;
+    const double* vj = slot >= 0 ? cvp + (rj * kZSlots + slot) * 8 : nullptr;
     pair_eval<GRAD, PG, S, PART>(t, p, A, zj, R[3 * j], R[3 * j + 1], R[3 * j + 2], r4j, cnj,
-                                 gj, gw + rj * kRefStride, gw0 + rj * kRefStride, acc, tf, pf,
-                                 pk);
+                                 gj, gw + rj * kRefStride, gw0 + rj * kRefStride, vj, acc, tf,
+                                 pf, pk);
     if (ATMF) {
       ptr[j] = tf;
       ppr[j] = pf;
@@ -2218,6 +2304,18 @@ hipError_t launch_prep(hipStream_t s, const Inputs& in, const Work& w) {
 hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
                      const Inputs& in, const Work& w) {
   D4_LAUNCH_ROWS((k_cn<16>), (k_cn<32>), (k_cn<64>), s, w, t, p, in, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_zslots(hipStream_t s, const Inputs& in, const Work& w) {
+  hipLaunchKernelGGL(k_zslots, dim3((unsigned)w.B), dim3(64), 0, s, in, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_pairvec(hipStream_t s, const DevTables& t, const Work& w, const Inputs& in,
+                          int part) {
+  const unsigned grid = (unsigned)((w.B * w.N * 8 + 255) / 256);
+  hipLaunchKernelGGL(k_pairvec, dim3(grid), dim3(256), 0, s, t, w, in, part);
   return hipGetLastError();
 }
 
