@@ -291,3 +291,27 @@ def test_finite_differences_16_atoms():
             em = d4.dftd4(numbers, pm, charge, PARAM, cutoff=_cut()).sum()
             fd = ((ep - em) / (2 * h)).item()
             assert abs(fd - g[0, i, x].item()) < 1e-9
+
+
+def test_many_elements_vs_oracle():
+    """A molecule with more distinct elements than the pair passes' element
+    slots (40 elements: reference C6 blocks contracted per pair) batched with
+    an ordinary one (contraction vectors per element): energies, fused and
+    autograd gradients against the oracle."""
+    _say("40 distinct elements + a 30-atom molecule vs oracle")
+    rng = np.random.default_rng(470)
+    many = synth.Molecule(np.arange(1, 41, dtype=np.int64), synth.grid_positions(rng, 40, 3.0),
+                          0.0)
+    small = synth.molecule(rng, 30, synth.MIX_Z54, 2.8, (0,))
+    numbers, positions, charge = synth.pack([many, small], device="cuda")
+    pos = positions.clone().requires_grad_(True)
+    e = d4.dftd4(numbers, pos, charge, PARAM, cutoff=_cut())
+    (g,) = torch.autograd.grad(e.sum(), pos)
+    ef, gf = d4.dftd4_energy_gradient(numbers, positions, charge, PARAM, cutoff=_cut())
+    for b, m in enumerate((many, small)):
+        n = len(m.numbers)
+        eo, go = _oracle(m)
+        assert (e[b, :n].detach().cpu() - eo).abs().max().item() < E_TOL, b
+        assert (g[b, :n].cpu() - go).abs().max().item() < G_TOL, b
+        assert (ef[b, :n].cpu() - eo).abs().max().item() < E_TOL, b
+        assert (gf[b, :n].cpu() - go).abs().max().item() < G_TOL, b
