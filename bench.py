@@ -59,7 +59,35 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--requests", type=int, default=10000, help="serving_stream requests")
     ap.add_argument("--model", choices=("d4", "d4s"), default="d4")
+    ap.add_argument("--shapes", default="tuned",
+                    help="launch shapes: 'tuned' (the table tuned for the device's "
+                         "architecture shipped in the package, when there is one), "
+                         "'builtin', or the path of a table written by tad_dftd4_amd.tune")
     return ap.parse_args()
+
+
+def use_shapes(choice: str, dev):
+    """Activate the launch-shape table; returns its label for the config."""
+    import torch
+
+    from tad_dftd4_amd import tune
+
+    if choice == "builtin":
+        tune.activate(None)
+        return "built-in rules"
+    if choice == "tuned":
+        arch = torch.cuda.get_device_properties(dev).gcnArchName.split(":")[0]
+        path = ROOT / "tad_dftd4_amd" / f"shapes_{arch}.json"
+        if not path.exists():
+            tune.activate(None)
+            return f"built-in rules (no tuned table for {arch})"
+    else:
+        path = Path(choice)
+    tune.activate(tune.ShapeTable.load(path))
+    try:
+        return f"tuned table {path.relative_to(ROOT)}"
+    except ValueError:
+        return f"tuned table {path}"
 
 
 def traffic_from_profiles(workload: str):
@@ -207,8 +235,11 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    shapes = use_shapes(args.shapes, dev)
     if args.workload == "serving_stream":
-        print(json.dumps(serve_bench(args)), flush=True)
+        out = serve_bench(args)
+        out["config"]["launch_shapes"] = shapes
+        print(json.dumps(out), flush=True)
         return 0
     forces = FORCES[args.workload]
     mols = synth.workload(args.workload)
@@ -216,6 +247,9 @@ def main() -> int:
     B, N = numbers.shape
     model = D4SModel(numbers) if args.model == "d4s" else None
     kp = kernel_params(PARAM, Cutoff(disp2=CUT[0], disp3=CUT[1], cn_eeq=CUT[2]), model)
+    rows, tile = kp.shapes(N, forces)
+    rows = rows or (16 if N <= 160 else 32 if N <= 384 else 64)
+    tile = tile or (8 if forces else 16)
     ws = runtime.new_workspace(B, N, dev)
     energy = torch.empty((B, N), dtype=torch.float64, device=dev)
     grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
@@ -284,13 +318,15 @@ def main() -> int:
             "cutoffs_bohr": list(CUT),
             "damping": PARAM,
             "parallelism": "single GPU",
+            "launch_shapes": {"source": shapes, "row_lanes": rows,
+                              "atm_tile": None if N > 256 else tile},
         },
         "roofline": {
             "kernel": ("k_atm8s (three-body, each triple once, 8x8 pair tiles, group screening, "
                        "energy + gradient)" if N > 256 else
-                       "k_atm8 (three-body, each triple once, 8x8 pair tiles, energy + gradient)"
-                       if forces else
-                       "k_atm1 (three-body, each triple once, 16x16 pair tiles, energy)"),
+                       f"{'k_atm8' if tile == 8 else 'k_atm1'} (three-body, each triple once, "
+                       f"{tile}x{tile} pair tiles, "
+                       f"{'energy + gradient' if forces else 'energy'})"),
             "bound": "mfma",
             "note": "FP64 VALU kernel (no MFMA); MI355X FP64 peak is 78.6 TFLOP/s for vector "
                     "and matrix alike",

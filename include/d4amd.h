@@ -46,6 +46,16 @@ extern "C" {
 #define D4_FLAG_ATM16 64   /* testing: one-pass three-body kernel with 16 x 16 pair tiles */
 #define D4_FLAG_PAIR_OWNER 128 /* testing: large systems take the pair-owner three-body
                                   kernel (each triple from its three pairs) */
+/* Launch shapes (results agree to rounding; used by the shape tuner,
+ * tad_dftd4_amd/tune.py). A two-bit field selects the lanes per atom row of
+ * the CN, pair and chain passes; 0 chooses them from the padded length. */
+#define D4_FLAG_ROWS_16 256
+#define D4_FLAG_ROWS_32 512
+#define D4_FLAG_ROWS_64 768
+#define D4_FLAG_ROWS_MASK 768
+#define D4_FLAG_ATM8 1024  /* energy-only calls: one-pass three-body kernel with 8 x 8
+                              pair tiles (default 16 x 16; gradient calls use 8 x 8
+                              unless D4_FLAG_ATM16) */
 
 /* Element and reference-system tables, host memory, read once by
  * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference

@@ -29,6 +29,11 @@ D4_FLAG_FORCE_PAIR_TILES = 16
 D4_FLAG_D4S = 32
 D4_FLAG_ATM16 = 64
 D4_FLAG_PAIR_OWNER = 128
+D4_FLAG_ROWS_16 = 256
+D4_FLAG_ROWS_32 = 512
+D4_FLAG_ROWS_64 = 768
+D4_FLAG_ROWS_MASK = 768
+D4_FLAG_ATM8 = 1024
 
 SLOTS = ("prep", "cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain")
 D4_PROFILE_SLOTS = 1 << 16

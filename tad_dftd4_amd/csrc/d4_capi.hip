@@ -84,6 +84,9 @@ d4::Params to_params(const d4_params* p) {
   k.d4s = (p->flags & D4_FLAG_D4S) ? 1 : 0;
   k.atm16 = (p->flags & D4_FLAG_ATM16) ? 1 : 0;
   k.pair_owner = (p->flags & D4_FLAG_PAIR_OWNER) ? 1 : 0;
+  const unsigned rows = ((unsigned)p->flags & D4_FLAG_ROWS_MASK) >> 8;
+  k.row_lanes = rows == 0 ? 0 : 8 << rows;  // 1, 2, 3 -> 16, 32, 64
+  k.atm8 = (p->flags & D4_FLAG_ATM8) ? 1 : 0;
   return k;
 }
 

@@ -58,6 +58,8 @@ struct Params {
   int d4s;    // D4S model: Gaussian weights with pairwise factors
   int atm16;  // one-pass three-body kernel with 16 x 16 pair tiles (testing)
   int pair_owner;  // large systems: pair-owner three-body kernel (testing)
+  int row_lanes;   // lanes per atom row of the row passes (16, 32, 64; 0: by length)
+  int atm8;        // energy-only calls: one-pass three-body kernel with 8 x 8 tiles
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).
@@ -219,7 +221,9 @@ __host__ __device__ inline int one_pass_splits(int n, bool tile8) {
 
 // The gradient passes use the 8 x 8 one-pass kernel, energies alone the
 // 16 x 16 one (fewer, larger work-groups suit the lighter energy slots).
-inline bool one_pass_tile8(const Params& p, bool grad) { return grad && !p.atm16; }
+inline bool one_pass_tile8(const Params& p, bool grad) {
+  return grad ? !p.atm16 : p.atm8 != 0;
+}
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 

@@ -2275,9 +2275,9 @@ inline unsigned rows_grid_rl(const Work& w, int rl) {
 
 // Launch a row kernel instantiated for the row width of this batch (the
 // three instantiations are passed in parentheses).
-#define D4_LAUNCH_ROWS(K16, K32, K64, s, w, ...)                  \
+#define D4_LAUNCH_ROWS(K16, K32, K64, s, p, w, ...)               \
   do {                                                           \
-    const int rl_ = row_lanes((w).N);                           \
+    const int rl_ = (p).row_lanes ? (p).row_lanes : row_lanes((w).N); \
     const dim3 g_(rows_grid_rl((w), rl_));                       \
     if (rl_ == 16)                                               \
       hipLaunchKernelGGL(K16, g_, dim3(256), 0, s, __VA_ARGS__); \
@@ -2303,7 +2303,7 @@ hipError_t launch_prep(hipStream_t s, const Inputs& in, const Work& w) {
 
 hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
                      const Inputs& in, const Work& w) {
-  D4_LAUNCH_ROWS((k_cn<16>), (k_cn<32>), (k_cn<64>), s, w, t, p, in, w);
+  D4_LAUNCH_ROWS((k_cn<16>), (k_cn<32>), (k_cn<64>), s, p, w, t, p, in, w);
   return hipGetLastError();
 }
 
@@ -2331,13 +2331,13 @@ void launch_pairs_model(hipStream_t s, const DevTables& t, const Params& p, cons
                         const Work& w, bool grad, double* grad_out) {
   if (grad && p.pgrad)
     D4_LAUNCH_ROWS((k_pairs<true, true, S, 16, PART>), (k_pairs<true, true, S, 32, PART>),
-                   (k_pairs<true, true, S, 64, PART>), s, w, t, p, in, w, grad_out);
+                   (k_pairs<true, true, S, 64, PART>), s, p, w, t, p, in, w, grad_out);
   else if (grad)
     D4_LAUNCH_ROWS((k_pairs<true, false, S, 16, PART>), (k_pairs<true, false, S, 32, PART>),
-                   (k_pairs<true, false, S, 64, PART>), s, w, t, p, in, w, grad_out);
+                   (k_pairs<true, false, S, 64, PART>), s, p, w, t, p, in, w, grad_out);
   else
     D4_LAUNCH_ROWS((k_pairs<false, false, S, 16, PART>), (k_pairs<false, false, S, 32, PART>),
-                   (k_pairs<false, false, S, 64, PART>), s, w, t, p, in, w, grad_out);
+                   (k_pairs<false, false, S, 64, PART>), s, p, w, t, p, in, w, grad_out);
 }
 
 template <int PART>
@@ -2489,7 +2489,7 @@ hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, con
 
 hipError_t launch_chain(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, double* grad_out) {
-  D4_LAUNCH_ROWS((k_chain<16>), (k_chain<32>), (k_chain<64>), s, w, t, p, in, w, grad_out);
+  D4_LAUNCH_ROWS((k_chain<16>), (k_chain<32>), (k_chain<64>), s, p, w, t, p, in, w, grad_out);
   return hipGetLastError();
 }
 

@@ -17,6 +17,10 @@ from . import _lib
 from .tables import GA_DEFAULT, GC_DEFAULT, WF_DEFAULT, device_tables
 
 _CTX: dict[tuple, ctypes.c_void_p] = {}
+# Launch-shape table in use (``tune.activate``): consulted by calls whose
+# KernelParams leave both shapes at 0.
+_SHAPES = None
+_ROW_FLAGS = {0: 0, 16: _lib.D4_FLAG_ROWS_16, 32: _lib.D4_FLAG_ROWS_32, 64: _lib.D4_FLAG_ROWS_64}
 _CTX_LOCK = threading.Lock()
 _KEEPALIVE: list = []
 
@@ -102,8 +106,25 @@ class KernelParams:
     d4s: bool = False  # D4S model (pairwise Gaussian weighting factors)
     atm16: bool = False  # testing: one-pass three-body kernel with 16 x 16 pair tiles
     pair_owner: bool = False  # testing: large systems take the pair-owner three-body kernel
+    # launch shapes (results agree to rounding): lanes per atom row of the CN,
+    # pair and chain passes (16, 32, 64; 0: by padded length) and the pair tile
+    # of the one-pass three-body kernel (8, 16; 0: 16 for energies, 8 with
+    # gradients). With both at 0 an activated shape table decides.
+    row_lanes: int = 0
+    atm_tile: int = 0
 
-    def c(self) -> _lib.Params:
+    def shapes(self, natoms: int | None = None, grad: bool = False) -> tuple[int, int]:
+        """(row_lanes, atm_tile) for a call on a batch padded to ``natoms``."""
+        if self.row_lanes == 0 and self.atm_tile == 0 and _SHAPES is not None and natoms:
+            return _SHAPES.lookup(grad, natoms)
+        return self.row_lanes, self.atm_tile
+
+    def c(self, natoms: int | None = None, grad: bool = False) -> _lib.Params:
+        rows, tile = self.shapes(natoms, grad)
+        if rows not in _ROW_FLAGS:
+            raise ValueError(f"row_lanes must be 0, 16, 32 or 64, got {rows}")
+        if tile not in (0, 8, 16):
+            raise ValueError(f"atm_tile must be 0, 8 or 16, got {tile}")
         flags = (
             (_lib.D4_FLAG_S10 if self.s10_on else 0)
             | (_lib.D4_FLAG_FORCE_LU if self.force_lu else 0)
@@ -111,8 +132,10 @@ class KernelParams:
             | (_lib.D4_FLAG_FORCE_SORT if self.force_sort else 0)
             | (_lib.D4_FLAG_FORCE_PAIR_TILES if self.force_pair_tiles else 0)
             | (_lib.D4_FLAG_D4S if self.d4s else 0)
-            | (_lib.D4_FLAG_ATM16 if self.atm16 else 0)
+            | (_lib.D4_FLAG_ATM16 if self.atm16 or tile == 16 else 0)
             | (_lib.D4_FLAG_PAIR_OWNER if self.pair_owner else 0)
+            | _ROW_FLAGS[rows]
+            | (_lib.D4_FLAG_ATM8 if tile == 8 else 0)
         )
         return _lib.Params(
             self.s6, self.s8, self.s10, self.s9, self.a1, self.a2, self.alp,
@@ -149,7 +172,7 @@ def forward(numbers, positions, charge, kp: KernelParams, q=None, r4r2=None,
         ws = new_workspace(B, N, dev)
     if energy is None:
         energy = torch.empty((B, N), dtype=torch.float64, device=dev)
-    p = kp.c()
+    p = kp.c(N, grad=False)
     _lib.check(
         _lib.load().d4_forward(
             ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
@@ -175,7 +198,7 @@ def backward(numbers, positions, charge, kp: KernelParams, ws, grad_energy=None,
     gpar = torch.empty(7, dtype=torch.float64, device=dev) if params else None
     if grad_energy is not None:
         grad_energy = grad_energy.to(torch.float64).contiguous()
-    p = kp.c()
+    p = kp.c(N, grad=True)
     _lib.check(
         _lib.load().d4_backward(
             ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
@@ -199,7 +222,7 @@ def energy_gradient(numbers, positions, charge, kp: KernelParams, q=None, r4r2=N
         energy = torch.empty((B, N), dtype=torch.float64, device=dev)
     if grad is None:
         grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
-    p = kp.c()
+    p = kp.c(N, grad=True)
     _lib.check(
         _lib.load().d4_energy_gradient(
             ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
@@ -222,7 +245,7 @@ def properties(numbers, positions, charge, kp: KernelParams, ws=None):
     q = torch.empty((B, N), **f64)
     c6 = torch.empty((B, N, N), **f64)
     alpha = torch.empty((B, N), **f64)
-    p = kp.c()
+    p = kp.c(N, grad=False)
     _lib.check(
         _lib.load().d4_properties(
             ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge),
