@@ -147,7 +147,8 @@ def tune(device=None, buckets=BUCKETS, modes=MODES, count: int = 512, reps: int 
         kp = runtime.KernelParams(s8=1.2, cutoff_disp2=50.0, cutoff_disp3=25.0)
     prev = active()
     activate(None)
-    table = ShapeTable(device=torch.cuda.get_device_name(device))
+    arch = torch.cuda.get_device_properties(device).gcnArchName.split(":")[0]
+    table = ShapeTable(device=f"{torch.cuda.get_device_name(device)} ({arch})")
     try:
         for mode in modes:
             chosen = []

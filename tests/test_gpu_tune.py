@@ -8,6 +8,7 @@ the autograd path included, with the same results.
 """
 
 import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -131,3 +132,26 @@ def test_tuned_table_drives_calls():
     for a, b in ((e, e0), (g, g0), (ef, e0), (gf, g0)):
         np.testing.assert_allclose(a.detach().cpu().numpy(), b.detach().cpu().numpy(),
                                    rtol=0, atol=SAME)
+
+
+def test_shipped_table_drives_calls():
+    _say("shipped gfx950 table: energies and gradient of a batch padded to 90")
+    table = tune.ShapeTable.load(Path(tune.__file__).with_name("shapes_gfx950.json"))
+    mols = _mols()
+    e0, _ = _run(_kp(), mols, forces=False)
+    ef0, g0 = _run(_kp(), mols)
+    tune.activate(table)
+    try:
+        assert _kp().shapes(90, False) == table.lookup(False, 90)
+        e, _ = _run(_kp(), mols, forces=False)
+        ef, g = _run(_kp(), mols)
+    finally:
+        tune.activate(None)
+    np.testing.assert_allclose(e, e0, rtol=0, atol=SAME)
+    np.testing.assert_allclose(ef, ef0, rtol=0, atol=SAME)
+    np.testing.assert_allclose(g, g0, rtol=0, atol=SAME)
+    for i, m in enumerate(mols):
+        n = len(m.numbers)
+        oe, og = _oracle()[i]
+        assert np.abs(e[i, :n] - oe).max() <= E_TOL
+        assert np.abs(g[i, :n] - og).max() <= G_TOL

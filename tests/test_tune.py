@@ -52,6 +52,18 @@ def test_json_round_trip_and_validation(tmp_path):
         tune.ShapeTable.from_json('{"entries": {"hessian": [[64, 16, 8]]}}')
 
 
+def test_shipped_table_is_valid():
+    path = ROOT / "tad_dftd4_amd" / "shapes_gfx950.json"
+    table = tune.ShapeTable.load(path)
+    for mode in tune.MODES:
+        ent = table.entries[mode]
+        assert [n for n, _, _ in ent] == sorted(tune.BUCKETS)
+        for n_max, rows, tile in ent:
+            assert (rows, tile) in tune.candidates(n_max)
+            assert table.timings[mode][str(n_max)][f"{rows}x{tile}"] == min(
+                table.timings[mode][str(n_max)].values())
+
+
 def test_candidates():
     c = tune.candidates(120)
     assert c[0] == (0, 0) and len(c) == 1 + len(tune.ROWS) * len(tune.TILES)
