@@ -1002,12 +1002,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
   __shared__ double s_acc[kAtm1MaxN][NA];
 
   const int64_t b = blockIdx.x;
-  // block b runs on XCD b % 8: consecutive splits of a molecule go to
-  // different XCDs, which evens out the XCDs' shares of a batch of mixed sizes
-  const int64_t logical = b;
-  if (logical >= nblocks) return;
-  const int64_t m = logical / S;
-  const int s = (int)(logical - m * S);
+  // block b runs on XCD b % 8. Molecule m owns blocks m S .. m S + S - 1 and
+  // its split s runs in block m S + (s + r) mod S, on XCD (m + s) mod 8 (up to
+  // the wrap-around): consecutive splits of a molecule go to different XCDs,
+  // and split 0 (and the other low splits, which every molecule has) of
+  // consecutive molecules too. r = m mod S when S is a multiple of 8, else
+  // r = m (1 - S) mod 8 (both measured: the first is up to 20 % faster where
+  // it applies, where it does not it would undo the rotation m S gives).
+  if (b >= nblocks) return;
+  const int64_t m = b / S;
+  const int slot = (int)(b - m * S);
+#ifdef D4_ATM1_NOROT
+  const int s = slot;
+#else
+  const int r = (S & 7) == 0 ? (int)(m % S) : (int)(m % 8) * (((1 - S) % 8 + 8) % 8) % 8;
+  const int s = (slot + S - r % S) % S;
+#endif
   const int64_t N = w.N;
   const int n = w.natom[m];
   const int Sm = atm1_splits(n);  // this molecule's split count (S is the batch maximum)
