@@ -14,7 +14,9 @@ stream, and keeps the fastest per bucket in a ``ShapeTable``. The table
 saves to and loads from JSON; ``activate(table)`` makes every later call
 whose ``KernelParams`` leave both shapes at 0 take the table's choice for
 its padded length (the first bucket at least as long; longer batches keep
-the built-in rules).
+the built-in rules). A molecule's results are bitwise the same alone and in
+any batch whose padded length falls in the same bucket (the buckets are the
+serving loop's padded lengths); across buckets they agree to rounding.
 
     from tad_dftd4_amd import tune
     table = tune.tune(torch.device("cuda:0"))
@@ -38,7 +40,10 @@ from . import runtime, synth
 ROWS = (16, 32, 64)
 TILES = (8, 16)
 ONE_PASS_MAX = 256  # longest padded length the one-pass three-body kernels take
-BUCKETS = (32, 64, 96, 128, 160, 200, 256, 384, 512)
+# the padded lengths of the serving loop's buckets up to 512 atoms: a request
+# alone and the serving batch it joins then take the same table entry (so the
+# same shapes and the same bits)
+BUCKETS = (32, 64, 96, 128, 160, 192, 256, 384, 512)
 MODES = ("energy", "forces")
 
 

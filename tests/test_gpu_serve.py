@@ -1,14 +1,18 @@
 """
 Continuous batching: requests of mixed size and kind, admitted while other
-batches run, give exactly the result each molecule gets alone.
+batches run, give exactly the result each molecule gets alone, with the
+built-in launch shapes and with the shipped tuned shape table (whose buckets
+are the serving loop's padded lengths).
 """
+
+from pathlib import Path
 
 import numpy as np
 import pytest
 import torch
 
 from oracle import dense
-from tad_dftd4_amd import runtime, synth
+from tad_dftd4_amd import runtime, synth, tune
 from tad_dftd4_amd.cutoff import Cutoff
 from tad_dftd4_amd.disp import kernel_params
 from tad_dftd4_amd.serve import D4Server, Request, latency_stats
@@ -28,8 +32,21 @@ def _requests(count, seed=7):
     return reqs
 
 
-def test_stream_alone_vs_mix():
-    print("[case] serving stream, 120 requests, budget 64 MiB, 3 in flight", flush=True)
+@pytest.mark.parametrize("shapes", ["builtin", "shipped"])
+def test_stream_alone_vs_mix(shapes):
+    print(f"[case] serving stream, 120 requests, budget 64 MiB, 3 in flight, {shapes} shapes",
+          flush=True)
+    table = None
+    if shapes == "shipped":
+        table = tune.ShapeTable.load(Path(tune.__file__).with_name("shapes_gfx950.json"))
+    tune.activate(table)
+    try:
+        _alone_vs_mix()
+    finally:
+        tune.activate(None)
+
+
+def _alone_vs_mix():
     cut = Cutoff(disp2=50.0, disp3=25.0, cn_eeq=25.0)
     server = D4Server(PARAM, cut, memory_budget=64 << 20, max_inflight=3, max_batch=24)
     reqs = _requests(120)
