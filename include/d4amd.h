@@ -43,7 +43,8 @@ extern "C" {
 #define D4_FLAG_FORCE_PAIR_TILES 16 /* testing: the large-system three-body kernel at any size */
 #define D4_FLAG_D4S 32     /* D4S model: Gaussian weights with pairwise factors
                               (reference src/tad_dftd4/model/d4s.py:56-285) */
-#define D4_FLAG_ATM16 64   /* testing: one-pass three-body kernel with 16 x 16 pair tiles */
+#define D4_FLAG_ATM16 64   /* gradient calls: one-pass three-body kernel with 16 x 16 pair
+                              tiles (default 8 x 8; energy-only calls use 16 x 16) */
 #define D4_FLAG_PAIR_OWNER 128 /* testing: large systems take the pair-owner three-body
                                   kernel (each triple from its three pairs) */
 /* Launch shapes (results agree to rounding; used by the shape tuner,
