@@ -19,8 +19,10 @@ energy-only or energy + forces) arrive on the host at any time. The loop
 - reports each result when its batch's completion event has fired.
 
 The kernels give every molecule a result that does not depend on its batch
-neighbours or the padded length, so a request's result is the same alone and
-in the mix.
+neighbours, nor on the padded length within one of the buckets up to 192
+atoms, so such a request's result is bitwise the same alone and in the mix;
+from 193 atoms on the padded batch takes the spatial order and the tiled EEQ
+and the results agree with the request alone to rounding.
 """
 
 from __future__ import annotations

@@ -14,9 +14,10 @@ stream, and keeps the fastest per bucket in a ``ShapeTable``. The table
 saves to and loads from JSON; ``activate(table)`` makes every later call
 whose ``KernelParams`` leave both shapes at 0 take the table's choice for
 its padded length (the first bucket at least as long; longer batches keep
-the built-in rules). A molecule's results are bitwise the same alone and in
-any batch whose padded length falls in the same bucket (the buckets are the
-serving loop's padded lengths); across buckets they agree to rounding.
+the built-in rules). Up to 192 atoms a molecule's results are bitwise the
+same alone and in any batch whose padded length falls in the same bucket (the
+buckets are the serving loop's padded lengths); otherwise they agree to
+rounding (longer batches also take the spatial order and the tiled EEQ).
 
     from tad_dftd4_amd import tune
     table = tune.tune(torch.device("cuda:0"))

@@ -155,3 +155,35 @@ def test_shipped_table_drives_calls():
         oe, og = _oracle()[i]
         assert np.abs(e[i, :n] - oe).max() <= E_TOL
         assert np.abs(g[i, :n] - og).max() <= G_TOL
+
+
+@pytest.mark.parametrize("n_pad", [128, 160, 256])
+def test_energy_splits_on_rotated_xcds(n_pad):
+    """Batches whose 16 x 16 split count is a multiple of 8 (8, 16, 64 at 128,
+    160, 256 atoms), where each molecule's splits are rotated over the XCDs by
+    its index: every molecule's energies are bitwise those it gets alone at the
+    same padded length (the same splits on other XCDs) and, up to 160 atoms,
+    alone at its own length (another split count; longer batches take the
+    spatial order and the tiled EEQ, whose sums differ in rounding), and the
+    oracle's."""
+    _say(f"16x16 energies, batch padded to {n_pad}: alone vs batch, oracle")
+    rng = np.random.default_rng(903 + n_pad)
+    sizes = (12, 40, 75, n_pad - 5, n_pad)
+    mols = [synth.molecule(rng, n, synth.MIX_Z54, 2.8, (-1, 0, 1)) for n in sizes]
+    kp = _kp(0, 16)
+    eb, _ = _run(kp, mols, forces=False)
+    assert eb.shape[1] == n_pad
+    tp = {k: torch.tensor(v, dtype=F64) for k, v in PARAM.items()}
+    for i, m in enumerate(mols):
+        n = len(m.numbers)
+        numbers, positions, charge = synth.pack([m], pad_to=n_pad, device="cuda")
+        ep, _ = runtime.forward(numbers, positions, charge, kp)
+        assert np.array_equal(eb[i], ep[0].cpu().numpy())
+        if n_pad <= 160:
+            ea, _ = _run(kp, [m], forces=False)
+            assert np.array_equal(eb[i, :n], ea[0, :n])
+        assert np.all(eb[i, n:] == 0.0)
+        if n <= 130:
+            oe = dense.dftd4(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge, tp,
+                             cutoff=CUT).numpy()
+            assert np.abs(eb[i, :n] - oe).max() <= E_TOL
