@@ -1534,7 +1534,10 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
   const double c3sq = p.cut3sq, alp = p.alp, a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
   // four steps per iteration (two with parameter gradients, whose extra
   // sums would not fit the registers), so every ring move below is static
-  constexpr int kUnroll = PG ? 2 : 4;
+#ifndef D4_ATM8_UNROLL
+#define D4_ATM8_UNROLL 4
+#endif
+  constexpr int kUnroll = PG ? 2 : D4_ATM8_UNROLL;
   D4_UNROLL(kUnroll)
   for (int k = 0; k < T8; ++k) {
     const int zl = tx ^ k;
@@ -1686,7 +1689,10 @@ __device__ __forceinline__ void atm8_store_pgrad(const Atm8Sums& u, int lane, do
 }
 
 template <bool ENERGY, bool GRAD, bool PG>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_atm8(
+#ifndef D4_ATM8_WAVES
+#define D4_ATM8_WAVES 2
+#endif
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVES))) void k_atm8(
     Params p, Inputs in, Work w, int S, int64_t nblocks) {
   constexpr int NA = kAtm1Acc;
   constexpr int T8 = kAtm8;
