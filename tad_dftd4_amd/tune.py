@@ -80,6 +80,8 @@ class ShapeTable:
             for n_max, rows, tile in lst:
                 if rows not in (0,) + ROWS or tile not in (0,) + TILES:
                     raise ValueError(f"invalid shape ({rows}, {tile}) in shape table")
+                if int(n_max) < 1:
+                    raise ValueError(f"invalid bucket length {n_max} in shape table")
                 rows_.append([int(n_max), int(rows), int(tile)])
             entries[mode] = sorted(rows_)
         return cls(d.get("device", ""), entries, d.get("timings", {}))
@@ -94,6 +96,8 @@ class ShapeTable:
 
 def activate(table: ShapeTable | None) -> None:
     """Use ``table`` for calls that leave the shapes at 0 (None: built-in rules)."""
+    if table is not None and not isinstance(table, ShapeTable):
+        raise TypeError(f"expected a ShapeTable or None, got {type(table).__name__}")
     runtime._SHAPES = table
 
 
@@ -148,6 +152,9 @@ def tune(device=None, buckets=BUCKETS, modes=MODES, count: int = 512, reps: int 
     (default: s8 = 1.2, a1 = 0.4, a2 = 5, s9 = 1, cutoffs 50/25/25); its own
     shapes are ignored. ``count`` molecules per bucket (fewer for long
     buckets, so that each bucket holds about the same number of atoms)."""
+    bad = [m for m in modes if m not in MODES]
+    if bad:
+        raise ValueError(f"unknown mode(s) {bad}; expected some of {MODES}")
     device = torch.device(device or "cuda:0")
     if kp is None:
         kp = runtime.KernelParams(s8=1.2, cutoff_disp2=50.0, cutoff_disp3=25.0)

@@ -50,6 +50,16 @@ def test_json_round_trip_and_validation(tmp_path):
         tune.ShapeTable.from_json('{"entries": {"energy": [[64, 16, 12]]}}')
     with pytest.raises(ValueError):
         tune.ShapeTable.from_json('{"entries": {"hessian": [[64, 16, 8]]}}')
+    with pytest.raises(ValueError):
+        tune.ShapeTable.from_json('{"entries": {"energy": [[0, 16, 8]]}}')
+
+
+def test_argument_checks():
+    with pytest.raises(TypeError):
+        tune.activate({"energy": [[64, 16, 8]]})
+    assert tune.active() is None
+    with pytest.raises(ValueError):
+        tune.tune("cpu", modes=("hessian",))
 
 
 def test_shipped_table_is_valid():
