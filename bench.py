@@ -9,12 +9,17 @@ grids; s6 = 1, s8 = 1.2, a1 = 0.40, a2 = 5.0 bohr, s9 = 1 (ATM on); cutoffs
 50 / 25 / 25 bohr. One step = atom energies and the gradient of the total
 energy for the whole batch, inputs resident in HBM (fused kernel sequence).
 
-Prints one JSON line with the throughput (molecules/s), a roofline record for
-the dominant kernel (three-body term, timed with HIP events on its launch
-stream), the plain per-molecule GPU path and the FP64 CPU restatement timed
-on the host on bounded samples of the same workload.
+Prints one JSON line with the throughput (molecules/s) of the step replayed
+from a captured HIP graph (the eager launch sequence's figure beside it), a
+roofline record for the dominant kernel (three-body term, timed with HIP
+events on its launch stream), the largest error of the timed step's output
+against the oracle on the CPU-baseline molecules, the plain per-molecule GPU
+path and the FP64 CPU restatement timed on the host on bounded samples of the
+same workload, the CPU time of the single 64-atom molecule (median of 20), and
+the serving stream's p50/p99 latency (``serve``).
 
 Other workloads (``--workload``), one JSON line each:
+  single_64        one 64-atom molecule (H/C/N/O), energy + gradient;
   batch_256_8_200  256 molecules, n ~ U{8..200}, energy only, one launch;
   cluster_5000     one 5000-atom cluster (H/C/N/O/S), energy + gradient;
   serving_stream   10,000 requests (n ~ lognormal, median 40, 4..2000 atoms,
@@ -57,6 +62,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="time budget of the CPU baseline sample")
     ap.add_argument("--no-baselines", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving-stream record of the headline workload")
     ap.add_argument("--requests", type=int, default=10000, help="serving_stream requests")
     ap.add_argument("--model", choices=("d4", "d4s"), default="d4")
     ap.add_argument("--shapes", default="tuned",
@@ -104,8 +111,10 @@ METRICS = {
     "batch_1024_20_120": "mol/s E+forces (20-120 atoms) batched vs plain; p50/p99 serve latency",
     "batch_256_8_200": "mol/s energy (8-200 atoms), one launch",
     "cluster_5000": "5000-atom cluster energy + forces evaluations/s",
+    "single_64": "single 64-atom molecule energy + forces, molecules/s",
 }
-FORCES = {"batch_1024_20_120": True, "batch_256_8_200": False, "cluster_5000": True}
+FORCES = {"batch_1024_20_120": True, "batch_256_8_200": False, "cluster_5000": True,
+          "single_64": True}
 
 
 def plain_gpu(mols, device, budget_s: float, workload: str = WORKLOAD) -> dict:
@@ -137,31 +146,104 @@ def plain_gpu(mols, device, budget_s: float, workload: str = WORKLOAD) -> dict:
             "the GPU"}
 
 
+def host_cpu() -> dict:
+    """Host CPU model and the threads the CPU legs use (the box's CPU share:
+    OMP_NUM_THREADS where the environment sets it, else every core)."""
+    import torch
+
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    torch.set_num_threads(threads)
+    return {"model": model, "threads": threads, "os_cpu_count": os.cpu_count()}
+
+
 def cpu_baseline(mols, budget_s: float, workload: str = WORKLOAD, forces: bool = True,
-                 model: str = "d4") -> dict:
+                 model: str = "d4") -> tuple[dict, list]:
+    """The oracle (dense FP64 restatement) on the host, one molecule per call,
+    over the first molecules of the workload until ``budget_s`` is spent.
+    Returns the record and the oracle's (energy, gradient) per molecule done."""
     import torch
 
     from oracle import dense
 
+    cpu = host_cpu()
     par = {k: torch.tensor(v, dtype=torch.float64) for k, v in PARAM.items()}
     cut = {"disp2": CUT[0], "disp3": CUT[1], "cn_eeq": CUT[2]}
+    outs = []
     t0 = time.perf_counter()
-    done = 0
     for m in mols:
         if forces:
-            dense.dftd4_with_grad(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge,
-                                  par, cutoff=cut, model=model)
+            e, g = dense.dftd4_with_grad(torch.tensor(m.numbers), torch.tensor(m.positions),
+                                         m.charge, par, cutoff=cut, model=model)
         else:
-            dense.dftd4(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge, par,
-                        cutoff=cut, model=model)
-        done += 1
+            e = dense.dftd4(torch.tensor(m.numbers), torch.tensor(m.positions), m.charge, par,
+                            cutoff=cut, model=model)
+            g = None
+        outs.append((e, g))
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
+    done = len(outs)
     what = "energy + autograd forces" if forces else "energy"
-    return {"value": done / dt, "unit": "molecules/s", "cores": torch.get_num_threads(),
-            "kind": "port", "sample": f"first {done} molecules of {workload}, {what}, FP64 "
-            "dense restatement on the host, one molecule per call"}
+    rec = {"value": done / dt, "unit": "molecules/s", "cores": cpu["threads"],
+           "kind": "port", "cpu_model": cpu["model"], "os_cpu_count": cpu["os_cpu_count"],
+           "sample": f"first {done} molecules of {workload}, {what}, FP64 dense restatement "
+                     f"on the host ({cpu['threads']} threads), one molecule per call"}
+    return rec, outs
+
+
+def cpu_single_64(runs: int = 20) -> dict:
+    """The single 64-atom molecule (H/C/N/O, neutral) on the host: energies and
+    the autograd gradient, 3 warm-up runs then the median of ``runs``."""
+    import statistics
+
+    import torch
+
+    from oracle import dense
+    from tad_dftd4_amd import synth
+
+    cpu = host_cpu()
+    (m,) = synth.workload("single_64")
+    par = {k: torch.tensor(v, dtype=torch.float64) for k, v in PARAM.items()}
+    cut = {"disp2": CUT[0], "disp3": CUT[1], "cn_eeq": CUT[2]}
+    nt, pt = torch.tensor(m.numbers), torch.tensor(m.positions)
+    times = []
+    for k in range(3 + runs):
+        t0 = time.perf_counter()
+        dense.dftd4_with_grad(nt, pt, m.charge, par, cutoff=cut)
+        if k >= 3:
+            times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"workload": "single_64", "median_ms": round(med * 1e3, 3),
+            "molecules_per_s": round(1.0 / med, 2), "runs": runs, "cores": cpu["threads"],
+            "cpu_model": cpu["model"], "kind": "port",
+            "what": "1 x 64 atoms, energy + autograd gradient, FP64 dense restatement"}
+
+
+def oracle_errors(outs, energy, grad) -> dict:
+    """Largest |E - E_oracle| and |g - g_oracle| over the molecules the CPU
+    baseline computed, from the timed step's own outputs."""
+    import torch
+
+    de = dg = 0.0
+    e = energy.cpu()
+    g = grad.cpu() if grad is not None else None
+    for b, (eo, go) in enumerate(outs):
+        n = eo.shape[0]
+        de = max(de, (e[b, :n] - eo).abs().max().item())
+        if go is not None and g is not None:
+            dg = max(dg, (g[b, :n] - go).abs().max().item())
+    rec = {"max_abs_dE": de, "molecules": len(outs), "tol_E": 1e-10}
+    if g is not None:
+        rec.update({"max_abs_dg": dg, "tol_g": 1e-9})
+    return rec
 
 
 def serve_bench(args) -> dict:
@@ -173,6 +255,7 @@ def serve_bench(args) -> dict:
     from tad_dftd4_amd.cutoff import Cutoff
     from tad_dftd4_amd.serve import D4Server, Request, latency_stats
 
+    host_cpu()
     rng = np.random.default_rng(4)
     sizes = synth.serving_sizes(rng, args.requests)
     reqs = []
@@ -208,6 +291,7 @@ def serve_bench(args) -> dict:
         "capacity_molecules_per_s": round(cap["molecules_per_s"], 1),
         "offered_rate_per_s": round(rate, 1),
         "batches": st["batches"],
+        "failed": st["failed"],
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "serving_stream", "requests": len(reqs),
@@ -259,34 +343,50 @@ def main() -> int:
             runtime.energy_gradient(numbers, positions, charge, kp, ws=ws, energy=energy,
                                     grad=grad)
         else:
-            runtime.forward(numbers, positions, charge, kp, ws=ws)
+            runtime.forward(numbers, positions, charge, kp, ws=ws, energy=energy)
+
+    def timed(fn, steps: int) -> float:
+        """Milliseconds of ``steps`` calls, bracketed by synchronisations and
+        HIP events on the current stream (where every launch sequence here
+        starts and, via the companion stream's join, ends)."""
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(steps):
+            fn()
+        stop.record()
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # eager launch sequence (host launches every kernel of every step)
+    ms_eager = timed(step, args.steps)
+    # the same steps with HIP events on the three-body kernel's launch stream
+    # bracketing that kernel alone (events around every kernel add ~5 % here)
     runtime.profile_read(dev)  # clear
-    # timed region: HIP events on the three-body kernel's launch stream
-    # bracket that kernel alone (events around every kernel add ~5 % here)
     runtime.profile_enable(dev, True, slots=("atm",))
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    for _ in range(args.steps):
-        step()
-    stop.record()
-    torch.cuda.synchronize()
-    ms_total = start.elapsed_time(stop)
+    timed(step, args.steps)
     runtime.profile_enable(dev, False)
     prof = runtime.profile_read(dev)
-    ms_step = ms_total / args.steps
-    value = B * args.steps / (ms_total / 1e3)
-    # per-kernel averages from a second, fully instrumented pass of the same steps
+    # per-kernel averages from a fully instrumented pass of the same steps
     runtime.profile_enable(dev, True)
     for _ in range(args.steps):
         step()
     runtime.profile_enable(dev, False)
     prof_all = runtime.profile_read(dev)
+    # the headline: the step captured once in a HIP graph (both streams'
+    # launch sequences) and replayed; K replays timed after W untimed ones
+    cap = runtime.CapturedStep(numbers, positions, charge, kp, forces=forces)
+    for _ in range(args.warmup):
+        cap.replay()
+    ms_total = timed(cap.replay, args.steps)
+    ms_step = ms_total / args.steps
+    value = B * args.steps / (ms_total / 1e3)
+    g_energy, g_grad = (cap.energy, cap.grad) if forces else (cap.energy, None)
+    same = bool(torch.equal(g_energy, energy) and (not forces or torch.equal(g_grad, grad)))
 
     work = metrics.workload_flops(mols, CUT[0], CUT[1], with_forces=forces)
     atm_ms, atm_n = prof["atm"]
@@ -327,9 +427,9 @@ def main() -> int:
                        f"{'k_atm8' if tile == 8 else 'k_atm1'} (three-body, each triple once, "
                        f"{tile}x{tile} pair tiles, "
                        f"{'energy + gradient' if forces else 'energy'})"),
-            "bound": "mfma",
-            "note": "FP64 VALU kernel (no MFMA); MI355X FP64 peak is 78.6 TFLOP/s for vector "
-                    "and matrix alike",
+            "bound": "fp64_valu",
+            "note": "FP64 vector-ALU kernel (MFMA unused: rocprof shows no MFMA instructions); "
+                    "MI355X FP64 peak is 78.6 TFLOP/s",
             "achieved": round(achieved, 3),
             "peak": metrics.PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -343,9 +443,14 @@ def main() -> int:
             "frac_of_fp64_peak": round(work["total"] / (ms_step / 1e3) / 1e12
                                        / metrics.PEAK_FP64_TFLOPS, 4),
         },
+        "eager": {"ms_per_step": round(ms_eager / args.steps, 4),
+                  "value": round(B * args.steps / (ms_eager / 1e3), 1),
+                  "note": "the same step launched kernel by kernel from the host"},
+        "timed_path": "HIP graph replay of the captured step (both streams)",
+        "graph_equals_eager": same,
         "kernel_avg_ms": kernel_ms,
-        "kernel_avg_note": "per-kernel HIP-event averages from a second pass of the same steps "
-                           "with every kernel bracketed; the timed region brackets only the "
+        "kernel_avg_note": "per-kernel HIP-event averages from an eager pass of the same steps "
+                           "with every kernel bracketed; the roofline pass brackets only the "
                            "three-body kernel",
     }
     if args.workload == "cluster_5000":
@@ -358,8 +463,22 @@ def main() -> int:
             pl = plain_gpu(mols, dev, args.plain_seconds, args.workload)
             out["plain_gpu"] = pl
             out["speedup_vs_plain"] = round(value / pl["value"], 1)
-        out["cpu_baseline"] = cpu_baseline(mols, args.cpu_seconds, args.workload, forces,
-                                           args.model)
+        out["cpu_baseline"], outs = cpu_baseline(mols, args.cpu_seconds, args.workload, forces,
+                                                 args.model)
+        out["oracle_check"] = oracle_errors(outs, g_energy, g_grad)
+        if args.workload in (WORKLOAD, "single_64"):
+            out["cpu_single_64"] = cpu_single_64()
+    if args.workload == WORKLOAD and args.model == "d4" and not args.no_serve:
+        sv = serve_bench(args)
+        out["serve"] = {k: sv[k] for k in (
+            "p50_ms", "p99_ms", "mean_ms", "molecules_per_s", "capacity_molecules_per_s",
+            "offered_rate_per_s", "batches", "failed")}
+        out["serve"]["requests"] = sv["config"]["requests"]
+        out["serve"]["atoms_min_max"] = sv["config"]["atoms_min_max"]
+        out["serve"]["atoms_median"] = sv["config"]["atoms_median"]
+        out["serve"]["workload"] = ("serving_stream: lognormal sizes (median 40, sigma 0.8, "
+                                    "4..2000 atoms), 70 % energy / 30 % energy + forces, "
+                                    "Poisson arrivals at 70 % of the closed-loop capacity")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -14,7 +14,12 @@
  * are zero.
  *
  * Errors: every function returns D4_OK (0) or a negative D4_ERR_* code and
- * never throws; d4_last_error() returns a thread-local message.
+ * never throws; d4_last_error() returns a thread-local message. Checks that
+ * need the contents of device buffers are made on the device and do not fail
+ * the call (the library never synchronises to read them): a molecule with an
+ * atomic number outside 0..103 gets zero outputs and status bit 2, a
+ * singular EEQ system status bit 1 (d4_molecule_status); callers that cannot
+ * vouch for their inputs read the status words after the call.
  */
 #ifndef D4AMD_H
 #define D4AMD_H
@@ -27,7 +32,7 @@ extern "C" {
 #endif
 
 #define D4_OK 0
-#define D4_ERR_INVALID (-1)   /* bad argument: null pointer, size, Z range */
+#define D4_ERR_INVALID (-1)   /* bad argument: null pointer, size, flags */
 #define D4_ERR_DEVICE (-2)    /* HIP runtime error */
 #define D4_ERR_NOMEM (-3)     /* device allocation failed */
 #define D4_ERR_WORKSPACE (-4) /* workspace smaller than d4_workspace_bytes */

@@ -16,7 +16,13 @@ energy-only or energy + forces) arrive on the host at any time. The loop
   outputs);
 - packs a batch with vectorised copies into the pinned buffer and moves it
   with one host-to-device copy, and the results with one copy back;
-- reports each result when its batch's completion event has fired.
+- reports each result when its batch's completion event has fired;
+- fails a bad request alone: a request with no atoms, atomic numbers
+  outside 0..103, positions of the wrong shape or not finite, or a footprint
+  above one slot's memory budget is answered at submission with a failed
+  ``Result`` (``error`` set); after a batch, a molecule whose status word
+  reports a singular EEQ system or whose energies are not finite gets a
+  failed result while the rest of its batch succeeds.
 
 The kernels give every molecule a result that does not depend on its batch
 neighbours, nor on the padded length within one of the buckets up to 192
@@ -42,6 +48,10 @@ from .disp import kernel_params
 __all__ = ["Request", "Result", "D4Server", "size_class"]
 
 _ALIGN = 256
+# molecule status bits of the workspace (include/d4amd.h, d4_molecule_status)
+STATUS_LU = 1
+STATUS_SINGULAR = 2
+STATUS_BAD_Z = 4
 
 
 def size_class(n: int) -> int:
@@ -70,11 +80,12 @@ class Request:
 @dataclass
 class Result:
     rid: int
-    energy: np.ndarray
+    energy: np.ndarray | None
     gradient: np.ndarray | None
     arrival: float
     done: float
     batch: int
+    error: str | None = None  # set for a failed request (energy and gradient None)
 
     @property
     def latency(self) -> float:
@@ -118,10 +129,41 @@ def _layout(nbatch: int, natoms: int, forces: bool) -> dict:
         off[name] = o
         o = _up(o + size)
     off["out_end"] = o
+    # host only: the molecules' status words, copied back with the outputs
+    off["status"] = o
+    off["host_end"] = _up(o + 4 * nbatch)
     off["ws"] = o
     off["ws_bytes"] = runtime.workspace_bytes(nbatch, natoms)
     off["dev_end"] = o + _up(off["ws_bytes"])
     return off
+
+
+def footprint(nbatch: int, natoms: int, forces: bool) -> int:
+    """Device bytes of a batch: inputs + outputs + workspace."""
+    return _layout(nbatch, natoms, forces)["dev_end"]
+
+
+def request_error(req: Request, slot_budget: int) -> str | None:
+    """Why ``req`` cannot be served by a slot of ``slot_budget`` device
+    bytes (None if it can): no atoms, atomic numbers outside 0..103,
+    positions of the wrong shape, non-finite inputs, or a footprint above the
+    budget even alone."""
+    numbers = np.asarray(req.numbers)
+    positions = np.asarray(req.positions)
+    n = len(numbers)
+    if n < 1:
+        return "a request needs at least one atom"
+    if numbers.ndim != 1 or not np.issubdtype(numbers.dtype, np.integer):
+        return "atomic numbers must be a 1-d integer array"
+    if numbers.min() < 0 or numbers.max() >= 104:
+        return "atomic numbers must lie in 0..103 (0 marks padding)"
+    if positions.shape != (n, 3):
+        return f"positions must have shape ({n}, 3), got {positions.shape}"
+    if not np.isfinite(positions).all() or not np.isfinite(req.charge):
+        return "positions and charge must be finite"
+    if footprint(1, size_class(n), req.forces) > slot_budget:
+        return f"a {n}-atom request exceeds the per-slot memory budget ({slot_budget} bytes)"
+    return None
 
 
 def _view(buf: torch.Tensor, off: int, dtype, shape) -> torch.Tensor:
@@ -151,10 +193,7 @@ class D4Server:
 
     # ------------------------------------------------------------ admission
 
-    @staticmethod
-    def footprint(nbatch: int, natoms: int, forces: bool) -> int:
-        """Device bytes of a batch: inputs + outputs + workspace."""
-        return _layout(nbatch, natoms, forces)["dev_end"]
+    footprint = staticmethod(footprint)
 
     def _admissible(self, npad: int, forces: bool, limit: int) -> int:
         lo, hi = 0, limit
@@ -166,9 +205,18 @@ class D4Server:
                 hi = mid - 1
         return lo
 
+    def validate(self, req: Request) -> str | None:
+        """Why ``req`` cannot be served (None if it can)."""
+        return request_error(req, self.slot_budget)
+
     def submit(self, req: Request) -> None:
-        if len(req.numbers) < 1:
-            raise ValueError("a request needs at least one atom")
+        """Queue ``req``; a request that cannot be served gets a failed
+        ``Result`` at once and does not enter a batch."""
+        why = self.validate(req)
+        if why is not None:
+            self.results.append(Result(req.rid, None, None, req.arrival, time.perf_counter(),
+                                       0, why))
+            return
         self.queues[(req.forces, size_class(len(req.numbers)))].append(req)
 
     def pending(self) -> int:
@@ -186,13 +234,13 @@ class D4Server:
         forces, npad = key
         q = self.queues[key]
         count = self._admissible(npad, forces, min(len(q), self.max_batch))
-        if count == 0:
+        if count == 0:  # submit() rejects such requests; kept as a guard
             raise MemoryError(f"a {npad}-atom request exceeds the per-slot memory budget "
                               f"({self.slot_budget} bytes)")
         reqs = [q.popleft() for _ in range(count)]
         slot = self._free.pop()
         lay = _layout(count, npad, forces)
-        slot.reserve(lay["dev_end"], lay["out_end"])
+        slot.reserve(lay["dev_end"], lay["host_end"])
         # pack into the pinned buffer: zero padding, then scatter every
         # request's atoms to (row, column) in one vectorised copy
         lens = np.fromiter((len(r.numbers) for r in reqs), dtype=np.int64, count=count)
@@ -223,6 +271,9 @@ class D4Server:
                 runtime.forward(nt, pt, ct, self.kp, ws=ws, energy=energy)
             lo, hi = lay["energy"], lay["out_end"]
             slot.host[lo:hi].copy_(slot.dev[lo:hi], non_blocking=True)
+            # status words: the first array of the workspace
+            slot.host[lay["status"]: lay["status"] + 4 * count].copy_(
+                ws[: 4 * count], non_blocking=True)
             slot.event = torch.cuda.Event()
             slot.event.record(slot.stream)
         self._bid += 1
@@ -251,11 +302,23 @@ class D4Server:
             if forces:
                 g = h[lay["grad"]: lay["grad"] + 24 * count * npad].view(np.float64)
                 g = g.reshape(count, npad, 3)
+            status = h[lay["status"]: lay["status"] + 4 * count].view(np.int32)
             for k, r in enumerate(slot.reqs):
                 n = len(r.numbers)
-                self.results.append(Result(r.rid, e[k, :n].copy(),
-                                           None if g is None else g[k, :n].copy(),
-                                           r.arrival, now, slot.bid))
+                ek = e[k, :n]
+                gk = None if g is None else g[k, :n]
+                why = None
+                if status[k] & STATUS_SINGULAR:
+                    why = "the EEQ system of this molecule is singular"
+                elif status[k] & STATUS_BAD_Z:
+                    why = "atomic numbers out of range"
+                elif not np.isfinite(ek).all() or (gk is not None and not np.isfinite(gk).all()):
+                    why = "non-finite result"
+                if why is None:
+                    self.results.append(Result(r.rid, ek.copy(), None if gk is None else gk.copy(),
+                                               r.arrival, now, slot.bid))
+                else:
+                    self.results.append(Result(r.rid, None, None, r.arrival, now, slot.bid, why))
                 added += 1
             slot.reqs = None
             self._free.append(slot)
@@ -300,9 +363,13 @@ class D4Server:
 
 
 def latency_stats(results: list[Result]) -> dict:
+    """Latency percentiles and throughput of the successful results."""
+    failed = sum(r.error is not None for r in results)
+    results = [r for r in results if r.error is None]
     lat = np.array([r.latency for r in results]) * 1e3
     span = max(r.done for r in results) - min(r.arrival for r in results)
     return {
+        "failed": failed,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(lat.mean()),

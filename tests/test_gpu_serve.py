@@ -101,3 +101,46 @@ def test_two_streams_share_a_context():
     torch.cuda.synchronize()
     for e1, g1, e2 in outs:
         assert torch.equal(e1, ea) and torch.equal(g1, ga) and torch.equal(e2, eb)
+
+
+def test_bad_requests_fail_alone():
+    """One request per failure kind in a stream of good ones: only those
+    fail. Two are rejected at submission (atomic number 120, a 3000-atom
+    request above a slot's budget); one with Z = 110 is queued past the
+    submission check, so the device's status word flags it inside its batch."""
+    print("[case] serving stream with bad requests", flush=True)
+    cut = Cutoff(disp2=50.0, disp3=25.0, cn_eeq=25.0)
+    server = D4Server(PARAM, cut, memory_budget=256 << 20, max_inflight=2, max_batch=16)
+    reqs = _requests(30, seed=21)
+    bad_z = Request(1000, np.array([6, 120, 1]), np.zeros((3, 3)) + np.arange(3)[:, None] * 2.0,
+                    0.0, forces=False)
+    rng = np.random.default_rng(2)
+    huge = synth.molecule(rng, 3000, synth.MIX_HCNO, 2.8, (0,))
+    too_big = Request(1001, huge.numbers, huge.positions, 0.0, forces=True)
+    planted = synth.molecule(rng, 20, synth.MIX_HCNO, 2.8, (0,))
+    planted.numbers[5] = 110
+    arrivals = np.cumsum(np.random.default_rng(4).exponential(2e-4, size=len(reqs)))
+    server.submit(bad_z)
+    server.submit(too_big)
+    server.queues[(False, 32)].append(Request(1002, planted.numbers, planted.positions, 0.0,
+                                              forces=False))
+    results = server.run_stream(reqs, arrivals)
+    by_id = {r.rid: r for r in server.results}
+    assert "0..103" in by_id[1000].error
+    assert "memory budget" in by_id[1001].error
+    assert "out of range" in by_id[1002].error
+    for r in (1000, 1001, 1002):
+        assert by_id[r].energy is None and by_id[r].gradient is None
+    good = [r for r in results if r.rid < 1000]
+    assert len(good) == len(reqs) and all(r.error is None for r in good)
+    kp = kernel_params(PARAM, cut, None)
+    for req in reqs:
+        numbers, positions, charge = synth.pack(
+            [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda")
+        if req.forces:
+            e, g, _ = runtime.energy_gradient(numbers, positions, charge, kp)
+            assert np.array_equal(by_id[req.rid].gradient, g[0].cpu().numpy())
+        else:
+            e, _ = runtime.forward(numbers, positions, charge, kp)
+        assert np.array_equal(by_id[req.rid].energy, e[0].cpu().numpy())
+    assert latency_stats(server.results)["failed"] == 3

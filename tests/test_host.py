@@ -147,3 +147,26 @@ def test_five_point_stencil():
         f = sum(ci * x**i for i, ci in enumerate(c))
         assert abs(disp._five_point(f).item() - c[1]) < 1e-9
     assert abs(disp._five_point(torch.sin(1.0 + x)).item() - np.cos(1.0)) < 1e-12
+
+
+def test_serving_request_validation():
+    """Requests the serving loop answers with a failed result at submission
+    (no GPU needed: the checks are on host arrays and workspace sizes)."""
+    from tad_dftd4_amd.serve import Request, footprint, request_error, size_class
+
+    pos = np.zeros((3, 3))
+    ok = Request(0, np.array([1, 6, 8]), pos, 0.0, forces=True)
+    budget = footprint(1, size_class(3), True)
+    assert request_error(ok, budget) is None
+    assert "at least one atom" in request_error(
+        Request(1, np.zeros(0, dtype=np.int64), np.zeros((0, 3)), 0.0, False), budget)
+    assert "0..103" in request_error(Request(2, np.array([1, 104, 8]), pos, 0.0, False), budget)
+    assert "0..103" in request_error(Request(3, np.array([-1, 6, 8]), pos, 0.0, False), budget)
+    assert "shape" in request_error(Request(4, np.array([1, 6, 8]), pos[:2], 0.0, False), budget)
+    bad = pos.copy()
+    bad[1, 2] = np.nan
+    assert "finite" in request_error(Request(5, np.array([1, 6, 8]), bad, 0.0, False), budget)
+    assert "finite" in request_error(Request(6, np.array([1, 6, 8]), pos, np.inf, False), budget)
+    assert "memory budget" in request_error(ok, budget - 1)
+    assert "integer" in request_error(Request(7, np.array([1.0, 6.0, 8.0]), pos, 0.0, False),
+                                      budget)
