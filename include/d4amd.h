@@ -86,6 +86,13 @@ typedef struct d4_tables {
   const double* rc6;   /* [zi][zj][a][b] reference C6, D4_MAX_Z^2 * 49 */
   const double* wfpair; /* [zi][zj] D4S Gaussian weighting factors, D4_MAX_Z^2
                            (src/tad_dftd4/data/wfpair.py); NULL: no D4S */
+  /* Needed by the d4_model_* functions only (NULL: those return
+   * D4_ERR_INVALID): */
+  const double* alpha;    /* reference polarisabilities alpha(i w) at the 23
+                             Casimir-Polder frequencies, D4_MAX_Z * D4_MAX_REF * 23,
+                             compacted like the other per-reference arrays */
+  const int32_t* refslot; /* D4_MAX_Z * D4_MAX_REF: the reference's slot (0..6) of
+                             each compacted reference system, -1 past nref[z] */
 } d4_tables;
 
 /* Damping parameters, model constants and cutoffs. Absent optional terms
@@ -168,6 +175,64 @@ int d4_properties(d4_context* ctx, void* stream, int64_t nbatch,
                   const d4_params* params, double* cn_out, double* q_out,
                   double* c6_out, double* alpha_out, void* workspace,
                   size_t workspace_bytes);
+
+/* Dispersion-model methods: the D4Model / D4SModel interface of the
+ * reference on the device. `params` supplies ga, gc, wf and D4_FLAG_D4S (the
+ * other fields are ignored); the context must hold `alpha` and `refslot`.
+ * Weights are in the reference's slot order: (nbatch, natoms, 7), or for
+ * D4S (nbatch, natoms, natoms, 7) with entry [m][n][r] the weight of atom
+ * n's reference r in its pair with atom m. No workspace is needed.
+ *
+ * d4_model_weights: weights for coordination numbers `cn` and charges `q`
+ * ((nbatch, natoms) each, NULL meaning zero, as the reference's defaults).
+ * dgwdcn_out (optional, D4 only): -d gw / d cn, the reference's analytic
+ * output, which carries the opposite sign of the derivative; dgwdq_out
+ * (optional): d gw / d q. Replaces D4Model.weight_references
+ * (src/tad_dftd4/model/d4.py:103-266) and D4SModel.weight_references
+ * (model/d4s.py:109-263). */
+int d4_model_weights(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                     const int64_t* numbers, const double* cn, const double* q,
+                     const d4_params* params, double* gw_out, double* dgwdcn_out,
+                     double* dgwdq_out);
+
+/* Vector-Jacobian product of d4_model_weights: dL/dcn and dL/dq
+ * ((nbatch, natoms), either may be NULL) of L = sum grad_gw * gw. Replaces
+ * autograd through weight_references (test/test_model/test_weights.py:
+ * 157-282). */
+int d4_model_weights_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                              const int64_t* numbers, const double* cn, const double* q,
+                              const d4_params* params, const double* grad_gw,
+                              double* grad_cn, double* grad_q);
+
+/* C6 of every atom pair from weights, c6_out (nbatch, natoms, natoms).
+ * Replaces D4Model.get_atomic_c6 (model/d4.py:268-289) and
+ * D4SModel.get_atomic_c6 (model/d4s.py:265-285). */
+int d4_model_atomic_c6(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                       const int64_t* numbers, const double* gw, const d4_params* params,
+                       double* c6_out);
+
+/* Vector-Jacobian product of d4_model_atomic_c6: grad_gw_out (the weights'
+ * shape) of L = sum grad_c6 * c6. */
+int d4_model_atomic_c6_backward(d4_context* ctx, void* stream, int64_t nbatch,
+                                int64_t natoms, const int64_t* numbers, const double* gw,
+                                const d4_params* params, const double* grad_c6,
+                                double* grad_gw_out);
+
+/* Polarisabilities from weights. static_only = 0: weighted polarisabilities
+ * at the 23 frequencies, out (nbatch, natoms, 23) (D4S: summed over the pair
+ * axis and divided by the molecule's atom count, as the reference), replacing
+ * get_weighted_pols (model/d4.py:291-306, model/d4s.py:287-313);
+ * static_only = 1: static polarisabilities, out (nbatch, natoms) (D4S:
+ * (nbatch, natoms, natoms), the reference's broadcast over the pair axis),
+ * replacing BaseModel.get_polarizabilities (model/base.py:286-302). */
+int d4_model_pols(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                  const int64_t* numbers, const double* gw, const d4_params* params,
+                  int static_only, double* out);
+
+/* Vector-Jacobian product of d4_model_pols: grad_gw_out of L = sum grad_out * out. */
+int d4_model_pols_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                           const int64_t* numbers, const d4_params* params, int static_only,
+                           const double* grad_out, double* grad_gw_out);
 
 /* Per-molecule status words of the last call that used `workspace`, copied
  * to host memory (synchronises the stream). Bit 0: EEQ solved by the

@@ -513,9 +513,28 @@ def _model_fns(model: str):
 
 
 def get_polarizabilities(numbers: Tensor, gw: Tensor) -> Tensor:
-    """``BaseModel.get_polarizabilities`` (``model/base.py:286-302``)."""
+    """``BaseModel.get_polarizabilities`` (``model/base.py:286-302``); with
+    D4S pair weights (..., m, n, 7) the einsum broadcasts over the pair axis
+    m (for a batch the reference's ellipsis broadcast fails; the pair axis
+    is inserted per molecule here, the single-molecule result)."""
+    alpha = get_alpha(numbers, gw.dtype)[..., 0]
+    if gw.dim() == numbers.dim() + 2:
+        alpha = alpha.unsqueeze(-3)
+    return torch.einsum("...nr,...nr->...n", gw, alpha)
+
+
+def get_weighted_pols(numbers: Tensor, gw: Tensor, model: str = "d4") -> Tensor:
+    """``D4Model.get_weighted_pols`` (``model/d4.py:291-306``): (..., n, 23);
+    ``D4SModel.get_weighted_pols`` (``model/d4s.py:287-313``): the pair
+    weights (..., m, n, 7) contracted over n and divided by the number of
+    real atoms of the molecule. (The reference divides by a (..., 1) count,
+    which broadcasts against (..., m, 23) only for a single molecule; for a
+    batch the division here is per molecule, the single-molecule result.)"""
     alpha = get_alpha(numbers, gw.dtype)
-    return torch.einsum("...nr,...nr->...n", gw, alpha[..., 0])
+    if model == "d4":
+        return torch.einsum("...nr,...nrw->...nw", gw, alpha)
+    nat = real_atoms(numbers).sum(dim=-1)[..., None, None]
+    return torch.einsum("...mnr,...nrw->...mw", gw, alpha) / nat
 
 
 # --------------------------------------------------------------------------

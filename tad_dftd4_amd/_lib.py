@@ -65,6 +65,8 @@ class Tables(ctypes.Structure):
         ("alpha0", _dp),
         ("rc6", _dp),
         ("wfpair", _dp),
+        ("alpha", _dp),
+        ("refslot", _ip),
     ]
 
 
@@ -115,6 +117,28 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, _i64, _i64, _vp, _vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp, _vp, _vp,
          ctypes.c_size_t],
+    ),
+    "d4_model_weights": (
+        ctypes.c_int,
+        [_vp, _vp, _i64, _i64, _vp, _vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp],
+    ),
+    "d4_model_weights_backward": (
+        ctypes.c_int,
+        [_vp, _vp, _i64, _i64, _vp, _vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp],
+    ),
+    "d4_model_atomic_c6": (
+        ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp, ctypes.POINTER(Params), _vp],
+    ),
+    "d4_model_atomic_c6_backward": (
+        ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp, ctypes.POINTER(Params), _vp, _vp],
+    ),
+    "d4_model_pols": (
+        ctypes.c_int,
+        [_vp, _vp, _i64, _i64, _vp, _vp, ctypes.POINTER(Params), ctypes.c_int, _vp],
+    ),
+    "d4_model_pols_backward": (
+        ctypes.c_int,
+        [_vp, _vp, _i64, _i64, _vp, ctypes.POINTER(Params), ctypes.c_int, _vp, _vp],
     ),
     "d4_molecule_status": (ctypes.c_int, [_vp, _i64, _vp, _ip]),
     "d4_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),

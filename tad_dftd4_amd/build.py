@@ -23,7 +23,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libd4amd.so"
-SOURCES = [CSRC / "d4_kernels.hip", CSRC / "d4_eeq.hip", CSRC / "d4_sort.hip", CSRC / "d4_capi.hip"]
+SOURCES = [CSRC / "d4_kernels.hip", CSRC / "d4_eeq.hip", CSRC / "d4_sort.hip",
+           CSRC / "d4_model.hip", CSRC / "d4_capi.hip"]
 HEADERS = [CSRC / "d4_internal.h", PKG.parent / "include" / "d4amd.h"]
 
 

@@ -30,6 +30,7 @@ struct d4_context {
   void* mem;
   d4::DevTables t;
   bool has_wfpair = false;  // D4S factors supplied at creation
+  bool has_model = false;   // alpha and refslot supplied (d4_model_* functions)
   std::mutex aux_mu;
   std::unordered_map<hipStream_t, d4_aux> aux;
   // diagnostics: per-kernel event timing
@@ -328,8 +329,9 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   const size_t nrc6 = size_t(D4_MAX_Z) * D4_MAX_Z * D4_MAX_REF * D4_MAX_REF;
   const size_t nwf = size_t(D4_MAX_Z) * D4_MAX_Z;
   const size_t npk = 4 * nwf;  // element-pair constants, filled on the device
-  const size_t ndbl = 10 * ne + 3 * nr + nrc6 + nwf + npk;
-  const size_t nint = ne + nr;
+  const size_t nal = nr * 23;  // alpha(i w) of every compacted reference system
+  const size_t ndbl = 10 * ne + 3 * nr + nrc6 + nwf + npk + nal;
+  const size_t nint = ne + 3 * nr;  // nref, ngw, refslot, refcomp
   std::vector<double> hd;
   hd.reserve(ndbl);
   for (const double* p : dsrc) hd.insert(hd.end(), p, p + ne);
@@ -342,12 +344,33 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   else
     hd.insert(hd.end(), nwf, 0.0);
   hd.insert(hd.end(), npk, 0.0);
+  const bool model = tb->alpha != nullptr && tb->refslot != nullptr;
+  if (model)
+    hd.insert(hd.end(), tb->alpha, tb->alpha + nal);
+  else
+    hd.insert(hd.end(), nal, 0.0);
   std::vector<int32_t> hi;
   hi.insert(hi.end(), tb->nref, tb->nref + ne);
   hi.insert(hi.end(), tb->ngw, tb->ngw + nr);
   for (size_t z = 0; z < ne; ++z)
     if (hi[z] < 0 || hi[z] > D4_MAX_REF)
       return set_error(D4_ERR_INVALID, "nref out of range");
+  // refslot (compact -> reference slot) and its inverse refcomp
+  std::vector<int32_t> slot(nr, -1), comp(nr, -1);
+  if (model) {
+    for (size_t z = 0; z < ne; ++z)
+      for (int c = 0; c < D4_MAX_REF; ++c) {
+        const int32_t r = tb->refslot[z * D4_MAX_REF + c];
+        if (c < hi[z]) {
+          if (r < 0 || r >= D4_MAX_REF || comp[z * D4_MAX_REF + r] != -1)
+            return set_error(D4_ERR_INVALID, "refslot is not a map onto distinct slots");
+          slot[z * D4_MAX_REF + c] = r;
+          comp[z * D4_MAX_REF + r] = c;
+        }
+      }
+  }
+  hi.insert(hi.end(), slot.begin(), slot.end());
+  hi.insert(hi.end(), comp.begin(), comp.end());
 
   int prev = 0;
   D4_TRY(hipGetDevice(&prev), "hipGetDevice");
@@ -372,6 +395,7 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   ctx->device = device;
   ctx->mem = mem;
   ctx->has_wfpair = tb->wfpair != nullptr;
+  ctx->has_model = model;
   const double* d = (const double*)mem;
   d4::DevTables& t = ctx->t;
   t.rcov = d + 0 * ne;
@@ -390,9 +414,12 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
   t.rc6 = t.alpha0 + nr;
   t.wfpair = t.rc6 + nrc6;
   t.pairk = t.wfpair + nwf;
+  t.alpha = t.pairk + npk;
   const int32_t* ip = (const int32_t*)(d + ndbl);
   t.nref = ip;
   t.ngw = ip + ne;
+  t.refslot = t.ngw + nr;
+  t.refcomp = t.refslot + nr;
   e = d4::launch_pair_consts(nullptr, t, const_cast<double*>(t.pairk));
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   (void)hipSetDevice(prev);
@@ -537,6 +564,120 @@ int d4_properties(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   if (q_out) D4_TRY(d4::launch_copy(c.s, c.w.q, q_out, nbatch * natoms), "copy charges");
   if (cn_out)
     D4_TRY(d4::launch_copy(c.s, c.w.cn, cn_out, nbatch * natoms), "copy coordination numbers");
+  return D4_OK;
+}
+
+// ------------------------------------------------------- model methods
+
+namespace {
+
+int check_model(const d4_context* ctx, int64_t nbatch, int64_t natoms, const int64_t* numbers,
+                const d4_params* params, d4::ModelCall* mc) {
+  if (ctx == nullptr) return set_error(D4_ERR_INVALID, "context is NULL");
+  if (params == nullptr) return set_error(D4_ERR_INVALID, "params is NULL");
+  if (nbatch < 0 || natoms < 0)
+    return set_error(D4_ERR_INVALID, "nbatch and natoms must be non-negative");
+  if (!ctx->has_model)
+    return set_error(D4_ERR_INVALID,
+                     "the context was created without the alpha and refslot tables");
+  if ((params->flags & D4_FLAG_D4S) && !ctx->has_wfpair)
+    return set_error(D4_ERR_INVALID, "D4_FLAG_D4S needs the wfpair table at context creation");
+  if (nbatch > 0 && natoms > 0 && numbers == nullptr)
+    return set_error(D4_ERR_INVALID, "numbers is NULL");
+  mc->B = nbatch;
+  mc->N = natoms;
+  mc->numbers = numbers;
+  mc->ga = params->ga;
+  mc->gc = params->gc;
+  mc->wf = params->wf;
+  mc->d4s = (params->flags & D4_FLAG_D4S) ? 1 : 0;
+  return D4_OK;
+}
+
+}  // namespace
+
+int d4_model_weights(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                     const int64_t* numbers, const double* cn, const double* q,
+                     const d4_params* params, double* gw_out, double* dgwdcn_out,
+                     double* dgwdq_out) {
+  d4::ModelCall mc;
+  int rc = check_model(ctx, nbatch, natoms, numbers, params, &mc);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (gw_out == nullptr) return set_error(D4_ERR_INVALID, "gw_out is NULL");
+  if (mc.d4s && dgwdcn_out)
+    return set_error(D4_ERR_INVALID, "the D4S model has no analytic CN derivative of its "
+                                     "weights (as the reference, model/d4s.py:252-256)");
+  D4_TRY(d4::launch_model_weights((hipStream_t)stream, ctx->t, mc, cn, q, gw_out, dgwdcn_out,
+                                  dgwdq_out, nullptr, nullptr, nullptr),
+         "model weights");
+  return D4_OK;
+}
+
+int d4_model_weights_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                              const int64_t* numbers, const double* cn, const double* q,
+                              const d4_params* params, const double* grad_gw,
+                              double* grad_cn, double* grad_q) {
+  d4::ModelCall mc;
+  int rc = check_model(ctx, nbatch, natoms, numbers, params, &mc);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (grad_gw == nullptr) return set_error(D4_ERR_INVALID, "grad_gw is NULL");
+  if (grad_cn == nullptr && grad_q == nullptr) return D4_OK;
+  D4_TRY(d4::launch_model_weights((hipStream_t)stream, ctx->t, mc, cn, q, nullptr, nullptr,
+                                  nullptr, grad_gw, grad_cn, grad_q),
+         "model weights backward");
+  return D4_OK;
+}
+
+int d4_model_atomic_c6(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                       const int64_t* numbers, const double* gw, const d4_params* params,
+                       double* c6_out) {
+  d4::ModelCall mc;
+  int rc = check_model(ctx, nbatch, natoms, numbers, params, &mc);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (gw == nullptr || c6_out == nullptr)
+    return set_error(D4_ERR_INVALID, "gw and c6_out must be non-NULL");
+  D4_TRY(d4::launch_model_c6((hipStream_t)stream, ctx->t, mc, gw, c6_out), "model C6");
+  return D4_OK;
+}
+
+int d4_model_atomic_c6_backward(d4_context* ctx, void* stream, int64_t nbatch,
+                                int64_t natoms, const int64_t* numbers, const double* gw,
+                                const d4_params* params, const double* grad_c6,
+                                double* grad_gw_out) {
+  d4::ModelCall mc;
+  int rc = check_model(ctx, nbatch, natoms, numbers, params, &mc);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (gw == nullptr || grad_c6 == nullptr || grad_gw_out == nullptr)
+    return set_error(D4_ERR_INVALID, "gw, grad_c6 and grad_gw_out must be non-NULL");
+  D4_TRY(d4::launch_model_c6_bwd((hipStream_t)stream, ctx->t, mc, gw, grad_c6, grad_gw_out),
+         "model C6 backward");
+  return D4_OK;
+}
+
+int d4_model_pols(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                  const int64_t* numbers, const double* gw, const d4_params* params,
+                  int static_only, double* out) {
+  d4::ModelCall mc;
+  int rc = check_model(ctx, nbatch, natoms, numbers, params, &mc);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (gw == nullptr || out == nullptr)
+    return set_error(D4_ERR_INVALID, "gw and out must be non-NULL");
+  D4_TRY(d4::launch_model_pols((hipStream_t)stream, ctx->t, mc, static_only ? 1 : 0, gw, out),
+         "model polarisabilities");
+  return D4_OK;
+}
+
+int d4_model_pols_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
+                           const int64_t* numbers, const d4_params* params, int static_only,
+                           const double* grad_out, double* grad_gw_out) {
+  d4::ModelCall mc;
+  int rc = check_model(ctx, nbatch, natoms, numbers, params, &mc);
+  if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
+  if (grad_out == nullptr || grad_gw_out == nullptr)
+    return set_error(D4_ERR_INVALID, "grad_out and grad_gw_out must be non-NULL");
+  D4_TRY(d4::launch_model_pols_bwd((hipStream_t)stream, ctx->t, mc, static_only ? 1 : 0,
+                                   grad_out, grad_gw_out),
+         "model polarisabilities backward");
   return D4_OK;
 }
 

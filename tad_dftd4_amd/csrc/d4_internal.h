@@ -42,6 +42,18 @@ struct DevTables {
   const double* pairk;   // [zi][zj][4] element-pair constants (see pair_const)
   const int32_t* nref;
   const int32_t* ngw;
+  // model methods (d4_model.hip); null when the context has no such tables
+  const double* alpha;     // [z][c][23] reference polarisabilities alpha(i w), compact slots
+  const int32_t* refslot;  // [z][c] reference slot of compact slot c (-1: unused)
+  const int32_t* refcomp;  // [z][s] compact slot of reference slot s (-1: unused)
+};
+
+// One call of a dispersion-model method (d4_model.hip).
+struct ModelCall {
+  int64_t B, N;
+  const int64_t* numbers;
+  double ga, gc, wf;
+  int d4s;
 };
 
 // Kernel-side copy of the parameters with derived constants.
@@ -364,5 +376,22 @@ hipError_t launch_scatter_pairs(hipStream_t s, const Work& w, const double* src,
                                 double* dst);
 hipError_t launch_groups(hipStream_t s, const Inputs& in, const Work& w);
 hipError_t launch_groups8(hipStream_t s, const Inputs& in, const Work& w);
+
+// Dispersion-model methods (d4_model.hip): weights from given CN and charges
+// (forward, or with grad_gw the backward to dL/dcn, dL/dq), C6 from given
+// weights and its backward, weighted (23 frequencies) or static
+// polarisabilities and their backward.
+hipError_t launch_model_weights(hipStream_t s, const DevTables& t, const ModelCall& mc,
+                                const double* cn, const double* q, double* gw_out,
+                                double* dgwdcn_out, double* dgwdq_out, const double* grad_gw,
+                                double* grad_cn, double* grad_q);
+hipError_t launch_model_c6(hipStream_t s, const DevTables& t, const ModelCall& mc,
+                           const double* gw, double* c6_out);
+hipError_t launch_model_c6_bwd(hipStream_t s, const DevTables& t, const ModelCall& mc,
+                               const double* gw, const double* grad_c6, double* grad_gw);
+hipError_t launch_model_pols(hipStream_t s, const DevTables& t, const ModelCall& mc,
+                             int static_only, const double* gw, double* out);
+hipError_t launch_model_pols_bwd(hipStream_t s, const DevTables& t, const ModelCall& mc,
+                                 int static_only, const double* grad_out, double* grad_gw);
 
 }  // namespace d4

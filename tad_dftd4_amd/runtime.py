@@ -41,7 +41,7 @@ def context(device: torch.device, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT
         t = device_tables(float(ga), float(gc))
         arrays = [t.rcov, t.en, t.r4r2, t.eeq_chi, t.eeq_eta, t.eeq_kcn, t.eeq_rad,
                   t.gam, t.zeff, t.maxcn, t.refcn, t.refq, t.alpha0, t.rc6, t.nref, t.ngw,
-                  t.wfpair]
+                  t.wfpair, t.alpha, t.refslot]
         _KEEPALIVE.append(arrays)
         tb = _lib.Tables(
             rcov=_ptr(t.rcov, ctypes.c_double),
@@ -61,6 +61,8 @@ def context(device: torch.device, ga: float = GA_DEFAULT, gc: float = GC_DEFAULT
             alpha0=_ptr(t.alpha0, ctypes.c_double),
             rc6=_ptr(t.rc6, ctypes.c_double),
             wfpair=_ptr(t.wfpair, ctypes.c_double),
+            alpha=_ptr(t.alpha, ctypes.c_double),
+            refslot=_ptr(t.refslot, ctypes.c_int32),
         )
         out = ctypes.c_void_p()
         _lib.check(lib.d4_context_create(index, ctypes.byref(tb), ctypes.byref(out)),
@@ -253,6 +255,132 @@ def properties(numbers, positions, charge, kp: KernelParams, ws=None):
         "d4_properties",
     )
     return cn, q, c6, alpha
+
+
+# ------------------------------------------------------------ model methods
+
+
+def _model_params(ga: float, gc: float, wf: float, d4s: bool) -> _lib.Params:
+    return _lib.Params(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ga, gc, wf, 0.0, 0.0, 0.0, 0.0,
+                       _lib.D4_FLAG_D4S if d4s else 0, 0)
+
+
+def _f64(t: torch.Tensor | None, device) -> torch.Tensor | None:
+    if t is None:
+        return None
+    return t.detach().to(device=device, dtype=torch.float64).contiguous()
+
+
+def model_weights(numbers, cn, q, ga: float, gc: float, wf: float, d4s: bool,
+                  dgwdcn: bool = False, dgwdq: bool = False):
+    """Reference weights (B, N, 7) (D4S: (B, N, N, 7)) for ``numbers`` (B, N)
+    int64 on the device and optional cn, q (B, N); with ``dgwdcn``/``dgwdq``
+    also the reference's analytic derivatives (-d gw/d cn, d gw/d q).
+    Returns (gw, dgwdcn or None, dgwdq or None)."""
+    dev = numbers.device
+    B, N = numbers.shape
+    ctx = context(dev, ga, gc)
+    shape = (B, N, N, 7) if d4s else (B, N, 7)
+    f64 = dict(dtype=torch.float64, device=dev)
+    gw = torch.empty(shape, **f64)
+    dc = torch.empty(shape, **f64) if dgwdcn else None
+    dq = torch.empty(shape, **f64) if dgwdq else None
+    cn, q = _f64(cn, dev), _f64(q, dev)
+    p = _model_params(ga, gc, wf, d4s)
+    _lib.check(
+        _lib.load().d4_model_weights(ctx, _stream(dev), B, N, _vp(numbers), _vp(cn), _vp(q),
+                                     ctypes.byref(p), _vp(gw), _vp(dc), _vp(dq)),
+        "d4_model_weights",
+    )
+    return gw, dc, dq
+
+
+def model_weights_backward(numbers, cn, q, ga, gc, wf, d4s, grad_gw, want_cn=True,
+                           want_q=True):
+    dev = numbers.device
+    B, N = numbers.shape
+    ctx = context(dev, ga, gc)
+    f64 = dict(dtype=torch.float64, device=dev)
+    gcn = torch.empty((B, N), **f64) if want_cn else None
+    gq = torch.empty((B, N), **f64) if want_q else None
+    cn, q, grad_gw = _f64(cn, dev), _f64(q, dev), _f64(grad_gw, dev)
+    p = _model_params(ga, gc, wf, d4s)
+    _lib.check(
+        _lib.load().d4_model_weights_backward(ctx, _stream(dev), B, N, _vp(numbers), _vp(cn),
+                                              _vp(q), ctypes.byref(p), _vp(grad_gw), _vp(gcn),
+                                              _vp(gq)),
+        "d4_model_weights_backward",
+    )
+    return gcn, gq
+
+
+def model_atomic_c6(numbers, gw, ga, gc, d4s):
+    dev = numbers.device
+    B, N = numbers.shape
+    ctx = context(dev, ga, gc)
+    c6 = torch.empty((B, N, N), dtype=torch.float64, device=dev)
+    gw = _f64(gw, dev)
+    p = _model_params(ga, gc, 0.0, d4s)
+    _lib.check(
+        _lib.load().d4_model_atomic_c6(ctx, _stream(dev), B, N, _vp(numbers), _vp(gw),
+                                       ctypes.byref(p), _vp(c6)),
+        "d4_model_atomic_c6",
+    )
+    return c6
+
+
+def model_atomic_c6_backward(numbers, gw, ga, gc, d4s, grad_c6):
+    dev = numbers.device
+    B, N = numbers.shape
+    ctx = context(dev, ga, gc)
+    gw = _f64(gw, dev)
+    out = torch.empty_like(gw)
+    grad_c6 = _f64(grad_c6, dev)
+    p = _model_params(ga, gc, 0.0, d4s)
+    _lib.check(
+        _lib.load().d4_model_atomic_c6_backward(ctx, _stream(dev), B, N, _vp(numbers), _vp(gw),
+                                                ctypes.byref(p), _vp(grad_c6), _vp(out)),
+        "d4_model_atomic_c6_backward",
+    )
+    return out
+
+
+def _pols_shape(B, N, d4s, static_only):
+    if static_only:
+        return (B, N, N) if d4s else (B, N)
+    return (B, N, 23)
+
+
+def model_pols(numbers, gw, ga, gc, d4s, static_only):
+    dev = numbers.device
+    B, N = numbers.shape
+    ctx = context(dev, ga, gc)
+    out = torch.empty(_pols_shape(B, N, d4s, static_only), dtype=torch.float64, device=dev)
+    gw = _f64(gw, dev)
+    p = _model_params(ga, gc, 0.0, d4s)
+    _lib.check(
+        _lib.load().d4_model_pols(ctx, _stream(dev), B, N, _vp(numbers), _vp(gw),
+                                  ctypes.byref(p), int(static_only), _vp(out)),
+        "d4_model_pols",
+    )
+    return out
+
+
+def model_pols_backward(numbers, ga, gc, d4s, static_only, grad_out):
+    dev = numbers.device
+    B, N = numbers.shape
+    ctx = context(dev, ga, gc)
+    shape = (B, N, N, 7) if d4s else (B, N, 7)
+    out = torch.empty(shape, dtype=torch.float64, device=dev)
+    grad_out = _f64(grad_out, dev)
+    p = _model_params(ga, gc, 0.0, d4s)
+    _lib.check(
+        _lib.load().d4_model_pols_backward(ctx, _stream(dev), B, N, _vp(numbers),
+                                           ctypes.byref(p), int(static_only), _vp(grad_out),
+                                           _vp(out)),
+        "d4_model_pols_backward",
+    )
+    return out
 
 
 class CapturedStep:

@@ -74,9 +74,11 @@ class DeviceTables:
     refcn: np.ndarray
     refq: np.ndarray
     alpha0: np.ndarray
+    alpha: np.ndarray  # (Z, ref, 23) reference alpha(i w), compacted slots
     rc6: np.ndarray
     wfpair: np.ndarray  # D4S pairwise Gaussian weighting factors (Z, Z)
     perm: np.ndarray  # compacted slot -> original slot (-1: unused)
+    refslot: np.ndarray  # perm as int32 (the C ABI's refslot table)
 
 
 def r4r2_table() -> np.ndarray:
@@ -127,7 +129,9 @@ def device_tables(ga: float = GA_DEFAULT, gc: float = GC_DEFAULT) -> DeviceTable
         refcn=c(compact(t["refcovcn"])),
         refq=c(compact(t["clsq"])),
         alpha0=c(alpha[..., 0]),
+        alpha=c(alpha),
         rc6=c(rc6),
         wfpair=c(t["wfpair"][:MAX_Z, :MAX_Z]),
         perm=perm,
+        refslot=c(perm, np.int32),
     )

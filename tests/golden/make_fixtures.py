@@ -23,6 +23,10 @@ Sources (relative to the reference checkout):
   ``(nref, nat, nat)``) and ``c6_d4s``.
 - ``test/test_d4/samples.py`` LiH and SiH4 ``disp2_d4s`` (two-body energy
   with the D4S model).
+- ``test/test_model/samples.py`` MB16_43_01, MB16_43_02, MB16_43_03 ``q``,
+  ``gw``, ``c6`` (and the D4S ``gw_d4s``, ``c6_d4s`` where present); these
+  molecules' atomic numbers live with their geometries in tad-mctc (absent)
+  and are inferred by the tests from the values themselves.
 
 The LiH and SiH4 geometries live in tad-mctc (absent); they are restated
 below from tad-mctc's molecule set (``data/molecules.py``: LiH bond along z,
@@ -63,6 +67,13 @@ RESTATED_GEOMETRY = {
         ],
     },
 }
+
+
+# What the MB16-43 ``gw`` fixtures were computed from (reference
+# test/test_model/test_weights.py:79-91): 01 with the geometry's CN and q = 0,
+# 02 with CN = 0 and the fixture q, 03 with both; ``c6`` of all three is the
+# C6 matrix of the molecule at its CN and charges (test_c6.py:32-33 uses 03).
+MB16_GW_INPUTS = {"MB16_43_01": "cn, q=0", "MB16_43_02": "cn=0, q", "MB16_43_03": "cn, q"}
 
 
 def _strip(text: str) -> str:
@@ -167,6 +178,16 @@ def build(ref: Path) -> dict:
             "disp2_d4s": _entry(b, "disp2_d4s"),
             "gw_d4s": _entry(m, "gw_d4s"),
             "c6_d4s": _entry(m, "c6_d4s"),
+        }
+    for name in ("MB16_43_01", "MB16_43_02", "MB16_43_03"):
+        m = _block(models, name)
+        out[name] = {
+            "q": _entry(m, "q"),
+            "gw": _entry(m, "gw"),
+            "c6": _entry(m, "c6"),
+            "gw_d4s": _entry(m, "gw_d4s"),
+            "c6_d4s": _entry(m, "c6_d4s"),
+            "gw_inputs": MB16_GW_INPUTS[name],
         }
     out["param_tpssh"] = {
         "s6": 1.0, "s8": 1.85897750, "s9": 1.0, "s10": 0.0, "alp": 16.0,

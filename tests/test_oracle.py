@@ -444,3 +444,191 @@ def test_hessian_fixture(name):
     assert (h - ref).abs().max().item() < 1e-9
     assert (h - h.permute(2, 3, 0, 1)).abs().max().item() < 1e-18
     assert h.sum(dim=2).abs().max().item() < 1e-15
+
+
+# ------------------------------------------------------ MB16-43 fixtures
+#
+# test/test_model/samples.py holds charges, weights and C6 of three
+# 16-atom molecules of tad-mctc's MB16-43 set, whose atomic numbers and
+# geometries live in tad-mctc (absent). The atomic numbers are inferred from
+# the values (test_mb16_atomic_numbers_are_identified checks that each is
+# the only element that fits); the fixtures then pin the charge scaling
+# (hardness GAM and Z_eff) and the reference C6 of B, F, Na, Mg, Al, Cl as
+# well as H, Li, C, N, O, Si, S.
+
+MB16_Z = {
+    "MB16_43_01": [11, 1, 8, 1, 9, 1, 1, 8, 7, 1, 1, 17, 5, 5, 7, 13],
+    "MB16_43_02": [1, 16, 5, 8, 12, 1, 1, 1, 14, 1, 5, 3, 9, 1, 1, 16],
+    "MB16_43_03": [6, 8, 1, 3, 12, 13, 6, 1, 1, 1, 9, 16, 6, 1, 11, 1],
+}
+
+
+def _zeta_ref(numbers, q):
+    """zeta(q) of every reference system of every atom (model/base.py:
+    308-335), zero on unused slots."""
+    t = dense.tables()
+    zeff = t.zeff[numbers].unsqueeze(-1)
+    z = dense.zeta(t.gam[numbers].unsqueeze(-1) * dense.GC, t.clsq[numbers] + zeff,
+                   q.unsqueeze(-1) + zeff)
+    return torch.where(t.refc[numbers] > 0, z, torch.zeros_like(z))
+
+
+def _mb16(name):
+    z = torch.tensor(MB16_Z[name])
+    q = torch.tensor(GOLD[name]["q"], dtype=F64)
+    c6 = torch.tensor(GOLD[name]["c6"], dtype=F64).reshape(16, 16)
+    return z, q, _fixture_gw(name), c6
+
+
+def _mb16_01_weights_at_q(gw, z, q):
+    """The 01 weights were made at q = 0 (test_model/test_weights.py:79-81);
+    the Gaussian part does not depend on q, so the weights at the charges
+    are gw * zeta(q) / zeta(0) (model/d4.py:219-228)."""
+    z0 = _zeta_ref(z, torch.zeros_like(q))
+    zq = _zeta_ref(z, q)
+    return torch.where(z0 > 0, gw * zq / torch.where(z0 > 0, z0, torch.ones_like(z0)),
+                       torch.zeros_like(gw))
+
+
+def _mb16_value(name, zc, i, gw, q, c6):
+    """What the fixture pins for atom i if it had atomic number zc: the
+    relative C6 diagonal (01, 03) or the largest weight deviation (02)."""
+    t = dense.tables()
+    zt = torch.tensor([zc])
+    if name == "MB16_43_02":
+        w = dense.weight_references(zt, None, q[i: i + 1])[0]
+        return (w - gw[i]).abs().max().item()
+    g = gw[i: i + 1]
+    if name == "MB16_43_01":
+        g = _mb16_01_weights_at_q(g, zt, q[i: i + 1])
+    g = torch.where(t.refc[zt] > 0, g, torch.zeros_like(g))
+    v = (g[0] @ dense.get_refc6(zt)[0, 0] @ g[0]).item()
+    return abs(v / c6[i, i].item() - 1.0)
+
+
+@pytest.mark.parametrize("name", sorted(MB16_Z))
+def test_mb16_atomic_numbers_are_identified(name):
+    _, q, gw, c6 = _mb16(name)
+    refc = dense.tables().refc
+    for i, zi in enumerate(MB16_Z[name]):
+        used = gw[i] != 0
+        scores = []
+        for zc in range(1, 104):
+            if bool((used & (refc[zc] == 0)).any()):
+                continue  # weight on a reference system the element lacks
+            scores.append((_mb16_value(name, zc, i, gw, q, c6), zc))
+        scores.sort()
+        assert scores[0][1] == zi, (name, i, scores[:3])
+        assert scores[0][0] < 1e-6 and scores[1][0] > 1e-4, (name, i, scores[:3])
+
+
+def test_mb16_43_03_c6_from_fixture_weights():
+    """test/test_model/test_c6.py:32-58 with MB16_43_03 (weights at the
+    molecule's CN and charges): pins the reference C6 of every element pair
+    of C, O, H, Li, Mg, Al, F, S, Na."""
+    z, _, gw, c6 = _mb16("MB16_43_03")
+    out = dense.get_atomic_c6(dense.get_refc6(z), gw)
+    assert torch.allclose(out, c6, rtol=1e-10, atol=0)
+
+
+def test_mb16_43_01_c6_from_weights_and_charges():
+    """MB16_43_01's weights (q = 0) rescaled to the fixture charges give the
+    fixture C6: pins zeta(q) / zeta(0), i.e. GAM and Z_eff, and the
+    reference C6 of Na, H, O, F, N, Cl, B, Al (the Fortran weights carry
+    ~1e-7 relative, as the LiH/SiH4 weights do)."""
+    z, q, gw, c6 = _mb16("MB16_43_01")
+    out = dense.get_atomic_c6(dense.get_refc6(z), _mb16_01_weights_at_q(gw, z, q))
+    assert torch.allclose(out, c6, rtol=1e-6, atol=0)
+
+
+def test_mb16_43_02_weights_at_zero_cn():
+    """MB16_43_02's weights were made at CN = 0 with the fixture charges
+    (test/test_model/test_weights.py:84-86): they follow from the tables
+    alone. Pins the reference CNs, Gaussian counts and the charge scaling of
+    H, S, B, O, Mg, Si, Li, F to the reference's own tolerance (atol 1e-6,
+    test_weights.py:45)."""
+    z, q, gw, _ = _mb16("MB16_43_02")
+    w = dense.weight_references(z, None, q)
+    assert torch.allclose(w, gw, rtol=0, atol=1e-6)
+    assert (w - gw).abs().max().item() < 1e-7
+
+
+@pytest.mark.parametrize("name", sorted(MB16_Z))
+def test_mb16_weights_normalised(name):
+    """sum_r gw_r / zeta_r = 1 per atom (model/d4.py:200-207), with zeta at
+    the charges the weights were made with (q = 0 for 01)."""
+    z, q, gw, _ = _mb16(name)
+    qq = torch.zeros_like(q) if name == "MB16_43_01" else q
+    zz = _zeta_ref(z, qq)
+    s = torch.where(zz > 0, gw / torch.where(zz > 0, zz, torch.ones_like(zz)),
+                    torch.zeros_like(gw)).sum(-1)
+    assert torch.allclose(s, torch.ones_like(s), rtol=0, atol=1e-6)
+
+
+# ------------------------------------------------------- polarisabilities
+
+
+def _trapzd_pols(wp):
+    """(3/pi) sum_w cp_w a_i(w) a_j(w): utils.trapzd_noref (utils.py:33-94),
+    the identity the reference's own test uses
+    (test/test_model/test_models.py:60-76)."""
+    cp = dense.tables().cp_weights
+    return 3.0 / math.pi * torch.einsum("iw,jw,w->ij", wp, wp, cp)
+
+
+@pytest.mark.parametrize("name", ["LiH", "SiH4", "MB16_43_03", "MB16_43_01"])
+def test_weighted_pols_reproduce_fixture_c6(name):
+    """get_weighted_pols of the fixture weights through the Casimir-Polder
+    quadrature gives the fixture C6 matrix: a wrong weight, slot, element or
+    frequency column in the weighted polarisabilities fails this."""
+    if name.startswith("MB16"):
+        z, q, gw, c6 = _mb16(name)
+        if name == "MB16_43_01":
+            gw = _mb16_01_weights_at_q(gw, z, q)
+        tol = 1e-6 if name == "MB16_43_01" else 1e-10
+    else:
+        z, _ = _mol(name)
+        gw = _fixture_gw(name)
+        n = len(z)
+        c6 = torch.tensor(GOLD[name]["c6"], dtype=F64).reshape(n, n)
+        tol = 1e-10
+    wp = dense.get_weighted_pols(z, gw)
+    assert wp.shape == (len(z), 23)
+    assert torch.allclose(_trapzd_pols(wp), c6, rtol=tol, atol=0)
+
+
+def test_static_polarizabilities_are_the_zero_frequency_column():
+    """The Casimir-Polder weights (utils.py:53-78) are exactly the trapezoid
+    weights of dftd4's published imaginary-frequency grid (1e-6, 0.05, 0.1,
+    0.2, ..., 2.0, 2.5, 3, 4, 5, 7.5, 10 a.u.; w_0 = (f_1 - f_0)/2,
+    w_k = (f_{k+1} - f_{k-1})/2, w_22 = (f_22 - f_21)/2), so column k of
+    alpha(i w) is frequency f_k and column 0 (f_0 = 1e-6) is the static
+    limit. get_polarizabilities (model/base.py:286-302) is that column of
+    the weighted polarisabilities pinned above."""
+    f = [1e-6, 0.05, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.2, 1.4, 1.6, 1.8,
+         2.0, 2.5, 3.0, 4.0, 5.0, 7.5, 10.0]
+    trap = [(f[1] - f[0]) / 2] + [(f[k + 1] - f[k - 1]) / 2 for k in range(1, 22)]
+    trap.append((f[22] - f[21]) / 2)
+    cp = dense.tables().cp_weights
+    assert torch.allclose(torch.tensor(trap, dtype=F64), cp, rtol=0, atol=1e-15)
+    for name in ("LiH", "SiH4"):
+        z, _ = _mol(name)
+        gw = _fixture_gw(name)
+        assert torch.equal(dense.get_polarizabilities(z, gw), dense.get_weighted_pols(z, gw)[:, 0])
+
+
+def test_d4s_weighted_pols_average_over_partners():
+    """D4S weighted polarisabilities (model/d4s.py:287-313) sum the pair
+    weights over the partner axis n and divide by the atom count: with pair
+    weights that do not depend on the partner (gw[m, n] = gw_n) every atom m
+    gets the mean of the D4 weighted polarisabilities; a padding atom counts
+    in neither."""
+    z, _ = _mol("SiH4")
+    gw = _fixture_gw("SiH4")
+    n = len(z)
+    zp = torch.cat([z, torch.zeros(2, dtype=z.dtype)])
+    gwp = torch.cat([gw, torch.zeros(2, 7, dtype=F64)])
+    pair = gwp.unsqueeze(0).expand(n + 2, n + 2, 7)
+    wp = dense.get_weighted_pols(zp, pair, model="d4s")
+    mean = dense.get_weighted_pols(z, gw).sum(0) / n
+    assert torch.allclose(wp, mean.expand(n + 2, 23), rtol=1e-14, atol=0)
