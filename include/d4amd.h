@@ -143,15 +143,20 @@ int d4_forward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
  * grad_params: optional device array of 7 doubles that receives dL/ds6,
  * dL/ds8, dL/ds10, dL/ds9, dL/da1, dL/da2, dL/dalp (summed over the batch;
  * the s8 and s10 entries are meaningful when those terms are on).
+ * grad_q: optional (nbatch, natoms), receives dL/dq for caller-supplied
+ * charges (requires q_in; zero on padding atoms).
  * Replaces the autograd backward of dftd4() with respect to the positions
- * (reference test/test_grad/test_pos.py:185-187) and the damping parameters
- * (test/test_grad/test_param.py:40-86). */
+ * (reference test/test_grad/test_pos.py:185-187), the damping parameters
+ * (test/test_grad/test_param.py:40-86) and given charges (the reference
+ * differentiates through q, src/tad_dftd4/dispersion/base.py:392-407 ->
+ * model/d4.py:219-228). */
 int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
                 const int64_t* numbers, const double* positions,
                 const double* charge, const double* q_in,
                 const double* r4r2_in, const d4_params* params,
                 const double* grad_energy, double* grad_positions,
-                double* grad_params, void* workspace, size_t workspace_bytes);
+                double* grad_params, double* grad_q, void* workspace,
+                size_t workspace_bytes);
 
 /* Fused atom energies and the gradient of the total energy in one pass.
  * Replaces dftd4() followed by torch.autograd.grad(energy.sum(), positions)

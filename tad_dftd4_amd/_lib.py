@@ -106,7 +106,7 @@ SIGNATURES = {
     "d4_backward": (
         ctypes.c_int,
         [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp,
-         _vp, ctypes.c_size_t],
+         _vp, _vp, ctypes.c_size_t],
     ),
     "d4_energy_gradient": (
         ctypes.c_int,

@@ -487,12 +487,14 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
                 const int64_t* numbers, const double* positions, const double* charge,
                 const double* q_in, const double* r4r2_in, const d4_params* params,
                 const double* grad_energy, double* grad_positions, double* grad_params,
-                void* workspace, size_t workspace_bytes) {
+                double* grad_q, void* workspace, size_t workspace_bytes) {
   int rc = check_common(ctx, nbatch, natoms, numbers, positions, charge, params,
                         workspace, workspace_bytes);
   if (rc != D4_OK || nbatch == 0 || natoms == 0) return rc;
   if (grad_positions == nullptr)
     return set_error(D4_ERR_INVALID, "grad_positions is NULL");
+  if (grad_q != nullptr && q_in == nullptr)
+    return set_error(D4_ERR_INVALID, "grad_q needs caller-supplied charges (q_in)");
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, q_in, r4r2_in,
                      params, workspace);
   c.in.gE = grad_energy;
@@ -506,6 +508,14 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   D4_TRY(scatter_atoms(c, c.w.sG, grad_positions, 3), "scatter gradient");
   if (grad_params)
     D4_TRY(d4::launch_pgrad_reduce(c.s, c.p, c.in, c.w, grad_params), "parameter gradients");
+  if (grad_q) {
+    // with given charges the two-body pass's dL/dq is the whole derivative
+    // (the three-body term uses q = 0 and no EEQ adjoint runs)
+    if (c.sorted)
+      D4_TRY(scatter_atoms(c, c.w.dldq, grad_q, 1), "scatter charge gradient");
+    else
+      D4_TRY(d4::launch_copy(c.s, c.w.dldq, grad_q, nbatch * natoms), "copy charge gradient");
+  }
   return D4_OK;
 }
 

@@ -13,11 +13,13 @@ from hand-written backward kernels through ``torch.autograd``.
 
 Not supported (raise ``NotImplementedError``; there is no fallback):
 non-default ``cn_function``/``counting_function``/``damping_function``,
-models other than D4 and D4S with EEQ reference charges, gradients with respect to
-user-supplied charges. Gradients with respect to the damping
-parameters s6, s8, s10, s9, a1, a2 and alp (0-d tensors with
-``requires_grad``) come from the same backward kernels
-(``test/test_grad/test_param.py:40-86``).
+models other than D4 and D4S with EEQ reference charges, gradients with
+respect to user-supplied ``r4r2`` values, second derivatives involving
+user-supplied charges. Gradients with respect to the damping parameters s6,
+s8, s10, s9, a1, a2 and alp (0-d tensors with ``requires_grad``) and to
+user-supplied charges ``q`` (the reference differentiates through q,
+``dispersion/base.py:392-407`` -> ``model/d4.py:219-228``) come from the same
+backward kernels (``test/test_grad/test_param.py:40-86``).
 
 Second derivatives with respect to the positions (double backward, e.g.
 ``torch.autograd.functional.hessian``) are semi-analytic: a Hessian-vector
@@ -114,16 +116,26 @@ class _D4Energy(torch.autograd.Function):
     def backward(ctx, grad_energy):
         positions, numbers, charge, q, r4r2, ws = ctx.saved_tensors
         want = any(ctx.needs_input_grad[7:])
+        want_q = ctx.needs_input_grad[3]
         ge = grad_energy.contiguous()
+        gq = None
         if torch.is_grad_enabled() and ctx.needs_input_grad[0]:
             # create_graph: the gradient itself is differentiable (Hessian-
-            # vector products through _D4Grad); parameter gradients are
-            # first order only
+            # vector products through _D4Grad); parameter and charge
+            # gradients are first order only
+            if want_q:
+                raise NotImplementedError(
+                    "second derivatives involving user-supplied charges are not supported"
+                )
             grad = _D4Grad.apply(positions, ge, numbers, charge, q, r4r2, ctx.kp, ws)
             gpar = None
             if want:
                 _, gpar = runtime.backward(numbers, positions.detach(), charge, ctx.kp, ws,
                                            ge.detach(), q=q, r4r2=r4r2, params=True)
+        elif want_q:
+            grad, gpar, gq = runtime.backward(numbers, positions.detach(), charge, ctx.kp, ws,
+                                              ge.detach(), q=q, r4r2=r4r2, params=want,
+                                              charges=True)
         else:
             out = runtime.backward(numbers, positions.detach(), charge, ctx.kp, ws,
                                    ge.detach(), q=q, r4r2=r4r2, params=want)
@@ -135,7 +147,7 @@ class _D4Energy(torch.autograd.Function):
             else:
                 pgrads.append(gpar[runtime.PARAM_ORDER.index(key)].reshape(shape).to(dtype))
         grad_pos = grad if ctx.needs_input_grad[0] else None
-        return (grad_pos, None, None, None, None, None, None, *pgrads)
+        return (grad_pos, None, None, gq, None, None, None, *pgrads)
 
 
 def _stencil(numbers, positions, charge, kp, q, r4r2, disp, grad_energy=None):
@@ -247,7 +259,8 @@ def _prepare(numbers, positions, charge, model, rcov, r4r2, rvdw, q, cn_function
         pos = pos.contiguous()
     chg = torch.as_tensor(charge, dtype=torch.float64, device=dev)
     chg = torch.broadcast_to(chg, lead).reshape(-1).contiguous()
-    qq = None if q is None else q.detach().to(dev, torch.float64).reshape(-1, n).contiguous()
+    # q stays attached to its autograd graph (dL/dq from the backward kernels)
+    qq = None if q is None else q.to(dev, torch.float64).reshape(-1, n).contiguous()
     rr = None if r4r2 is None else r4r2.detach().to(dev, torch.float64).reshape(-1, n).contiguous()
     return model_inst, nums, pos, chg, qq, rr, lead, n
 
@@ -273,7 +286,12 @@ def dftd4(
     with respect to ``positions``.
 
     ``rcov`` and ``rvdw`` are validated and unused, as in the reference
-    (``dispersion/base.py:366-387``)."""
+    (``dispersion/base.py:366-387``). Also differentiable with respect to
+    the damping-parameter tensors in ``param`` and a given ``q``."""
+    if isinstance(r4r2, torch.Tensor) and r4r2.requires_grad and torch.is_grad_enabled():
+        raise NotImplementedError(
+            "gradients with respect to user-supplied r4r2 values are not supported"
+        )
     model_inst, nums, pos, chg, qq, rr, lead, n = _prepare(
         numbers, positions, charge, model, rcov, r4r2, rvdw, q, cn_function,
         counting_function, damping_function,

@@ -188,26 +188,31 @@ PARAM_ORDER = ("s6", "s8", "s10", "s9", "a1", "a2", "alp")
 
 
 def backward(numbers, positions, charge, kp: KernelParams, ws, grad_energy=None,
-             q=None, r4r2=None, params: bool = False):
+             q=None, r4r2=None, params: bool = False, charges: bool = False):
     """Gradient of L = sum(grad_energy * E) w.r.t. positions, (B, N, 3); with
     ``params`` also dL/d(s6, s8, s10, s9, a1, a2, alp) as a (7,) tensor
-    (order ``PARAM_ORDER``), returned as a second value."""
+    (order ``PARAM_ORDER``), returned as a second value; with ``charges``
+    (given ``q``) also dL/dq (B, N): then the result is (grad, gpar or None,
+    grad_q)."""
     _check_inputs(numbers, positions, charge)
     B, N = numbers.shape
     dev = positions.device
     ctx = context(dev, kp.ga, kp.gc)
     grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
     gpar = torch.empty(7, dtype=torch.float64, device=dev) if params else None
+    gq = torch.empty((B, N), dtype=torch.float64, device=dev) if charges else None
     if grad_energy is not None:
         grad_energy = grad_energy.to(torch.float64).contiguous()
     p = kp.c(N, grad=True)
     _lib.check(
         _lib.load().d4_backward(
             ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
-            _vp(r4r2), ctypes.byref(p), _vp(grad_energy), _vp(grad), _vp(gpar), _vp(ws),
-            ws.numel()),
+            _vp(r4r2), ctypes.byref(p), _vp(grad_energy), _vp(grad), _vp(gpar), _vp(gq),
+            _vp(ws), ws.numel()),
         "d4_backward",
     )
+    if charges:
+        return grad, gpar, gq
     return (grad, gpar) if params else grad
 
 

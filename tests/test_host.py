@@ -170,3 +170,13 @@ def test_serving_request_validation():
     assert "memory budget" in request_error(ok, budget - 1)
     assert "integer" in request_error(Request(7, np.array([1.0, 6.0, 8.0]), pos, 0.0, False),
                                       budget)
+
+
+def test_r4r2_gradient_is_refused():
+    """dftd4 raises before any launch when r4r2 requires grad (the backward
+    kernels do not form dE/dr4r2)."""
+    numbers = torch.tensor([6, 1])
+    positions = torch.zeros(2, 3, dtype=torch.float64)
+    r4r2 = torch.ones(2, dtype=torch.float64, requires_grad=True)
+    with pytest.raises(NotImplementedError):
+        d4.dftd4(numbers, positions, 0.0, {"a1": 0.4, "a2": 5.0}, r4r2=r4r2)
