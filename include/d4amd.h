@@ -169,6 +169,24 @@ int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch,
                        double* grad_positions, void* workspace,
                        size_t workspace_bytes);
 
+/* d4_energy_gradient without a workspace argument and without any device
+ * allocation: the workspace is a fixed scratch array of
+ * D4_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs on
+ * until the process ends (its address is looked up at d4_context_create).
+ * Batches whose d4_workspace_bytes exceeds it are rejected with
+ * D4_ERR_INVALID. Calls on one GPU share the scratch: calls that overlap in
+ * time on one GPU (other streams, other threads) race and must be
+ * serialised by the caller. No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+#define D4_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int d4_energy_gradient_noalloc(d4_context* ctx, void* stream, int64_t nbatch,
+                               int64_t natoms, const int64_t* numbers,
+                               const double* positions, const double* charge,
+                               const double* q_in, const double* r4r2_in,
+                               const d4_params* params, double* energy_out,
+                               double* grad_positions);
+
 /* Per-atom properties: D4 coordination numbers, charges, the C6 matrix
  * (nbatch, natoms, natoms) and static polarisabilities (nbatch, natoms).
  * Replaces get_properties() (reference src/tad_dftd4/disp.py:149-197).

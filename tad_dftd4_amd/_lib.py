@@ -34,6 +34,7 @@ D4_FLAG_ROWS_32 = 512
 D4_FLAG_ROWS_64 = 768
 D4_FLAG_ROWS_MASK = 768
 D4_FLAG_ATM8 = 1024
+D4_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 SLOTS = ("prep", "cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain")
 D4_PROFILE_SLOTS = 1 << 16
@@ -112,6 +113,10 @@ SIGNATURES = {
         ctypes.c_int,
         [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp,
          ctypes.c_size_t],
+    ),
+    "d4_energy_gradient_noalloc": (
+        ctypes.c_int,
+        [_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(Params), _vp, _vp],
     ),
     "d4_properties": (
         ctypes.c_int,

@@ -25,9 +25,17 @@ struct d4_aux {
   hipEvent_t fork = nullptr, join = nullptr;
 };
 
+// Scratch of d4_energy_gradient_noalloc: the workspace of that entry point,
+// a fixed array on every device the library runs on (held until the process
+// ends). Host code never uses this name as an address: the device address of
+// each GPU's copy comes from hipGetSymbolAddress at context creation.
+constexpr size_t kScratchBytes = D4_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 struct d4_context {
   int device;
   void* mem;
+  void* scratch_dev = nullptr;  // this device's copy of `scratch`
   d4::DevTables t;
   bool has_wfpair = false;  // D4S factors supplied at creation
   bool has_model = false;   // alpha and refslot supplied (d4_model_* functions)
@@ -44,6 +52,10 @@ struct d4_context {
 namespace {
 
 thread_local std::string g_error;
+
+// device -> address of its copy of `scratch`, looked up once per device
+std::mutex g_scratch_mu;
+std::unordered_map<int, void*> g_scratch_addr;
 
 int set_error(int code, const std::string& msg) {
   g_error = msg;
@@ -391,9 +403,28 @@ int d4_context_create(int device, const d4_tables* tb, d4_context** out) {
     (void)hipFree(mem);
     return hip_error(e, "upload tables");
   }
+  void* scratch_dev = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) {
+      scratch_dev = it->second;
+    } else {
+      // `device` is current here
+      e = hipGetSymbolAddress(&scratch_dev, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || scratch_dev == nullptr) {
+        (void)hipSetDevice(prev);
+        (void)hipFree(mem);
+        return hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                         "hipGetSymbolAddress(scratch)");
+      }
+      g_scratch_addr.emplace(device, scratch_dev);
+    }
+  }
   d4_context* ctx = new d4_context;
   ctx->device = device;
   ctx->mem = mem;
+  ctx->scratch_dev = scratch_dev;
   ctx->has_wfpair = tb->wfpair != nullptr;
   ctx->has_model = model;
   const double* d = (const double*)mem;
@@ -517,6 +548,30 @@ int d4_backward(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
       D4_TRY(d4::launch_copy(c.s, c.w.dldq, grad_q, nbatch * natoms), "copy charge gradient");
   }
   return D4_OK;
+}
+
+int d4_energy_gradient_noalloc(d4_context* ctx, void* stream, int64_t nbatch,
+                               int64_t natoms, const int64_t* numbers,
+                               const double* positions, const double* charge,
+                               const double* q_in, const double* r4r2_in,
+                               const d4_params* params, double* energy_out,
+                               double* grad_positions) {
+  if (ctx == nullptr) return set_error(D4_ERR_INVALID, "context is NULL");
+  if (nbatch < 0 || natoms < 0)
+    return set_error(D4_ERR_INVALID, "nbatch and natoms must be non-negative");
+  size_t need = 0;
+  d4_workspace_bytes(nbatch, natoms, &need);
+  if (need > kScratchBytes)
+    return set_error(D4_ERR_INVALID,
+                     "d4_energy_gradient_noalloc: this batch needs " + std::to_string(need) +
+                         " bytes of workspace, above its fixed scratch of " +
+                         std::to_string(kScratchBytes) +
+                         " bytes; call d4_energy_gradient with a workspace instead");
+  // the scratch holds no state between calls: every array a call reads is
+  // written earlier in the same call, as with a fresh workspace
+  return d4_energy_gradient(ctx, stream, nbatch, natoms, numbers, positions, charge, q_in,
+                            r4r2_in, params, energy_out, grad_positions, ctx->scratch_dev,
+                            kScratchBytes);
 }
 
 int d4_energy_gradient(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,

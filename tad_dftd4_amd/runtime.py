@@ -239,6 +239,30 @@ def energy_gradient(numbers, positions, charge, kp: KernelParams, q=None, r4r2=N
     return energy, grad, ws
 
 
+def energy_gradient_noalloc(numbers, positions, charge, kp: KernelParams, q=None, r4r2=None,
+                            energy=None, grad=None):
+    """``energy_gradient`` through d4_energy_gradient_noalloc: the library's
+    fixed per-device scratch (``_lib.D4_NOALLOC_SCRATCH_BYTES``) instead of a
+    workspace; no device allocation when ``energy`` and ``grad`` are given.
+    Calls that overlap on one GPU race on the scratch."""
+    _check_inputs(numbers, positions, charge)
+    B, N = numbers.shape
+    dev = positions.device
+    ctx = context(dev, kp.ga, kp.gc)
+    if energy is None:
+        energy = torch.empty((B, N), dtype=torch.float64, device=dev)
+    if grad is None:
+        grad = torch.empty((B, N, 3), dtype=torch.float64, device=dev)
+    p = kp.c(N, grad=True)
+    _lib.check(
+        _lib.load().d4_energy_gradient_noalloc(
+            ctx, _stream(dev), B, N, _vp(numbers), _vp(positions), _vp(charge), _vp(q),
+            _vp(r4r2), ctypes.byref(p), _vp(energy), _vp(grad)),
+        "d4_energy_gradient_noalloc",
+    )
+    return energy, grad
+
+
 def properties(numbers, positions, charge, kp: KernelParams, ws=None):
     """(cn, q, c6, alpha) as in get_properties."""
     _check_inputs(numbers, positions, charge)
