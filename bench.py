@@ -64,6 +64,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--no-baselines", action="store_true")
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the serving-stream record of the headline workload")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="profiling: run the three-body side on the caller's stream (no "
+                         "companion stream), so a kernel trace shows each kernel's own time")
     ap.add_argument("--requests", type=int, default=10000, help="serving_stream requests")
     ap.add_argument("--model", choices=("d4", "d4s"), default="d4")
     ap.add_argument("--shapes", default="tuned",
@@ -331,6 +334,7 @@ def main() -> int:
     B, N = numbers.shape
     model = D4SModel(numbers) if args.model == "d4s" else None
     kp = kernel_params(PARAM, Cutoff(disp2=CUT[0], disp3=CUT[1], cn_eeq=CUT[2]), model)
+    kp.one_stream = args.one_stream
     rows, tile = kp.shapes(N, forces)
     rows = rows or (16 if N <= 160 else 32 if N <= 384 else 64)
     tile = tile or (8 if forces else 16)
@@ -420,6 +424,7 @@ def main() -> int:
             "parallelism": "single GPU",
             "launch_shapes": {"source": shapes, "row_lanes": rows,
                               "atm_tile": None if N > 256 else tile},
+            "streams": "one (profiling)" if args.one_stream else "caller + companion",
         },
         "roofline": {
             "kernel": ("k_atm8s (three-body, each triple once, 8x8 pair tiles, group screening, "

@@ -62,6 +62,9 @@ extern "C" {
 #define D4_FLAG_ATM8 1024  /* energy-only calls: one-pass three-body kernel with 8 x 8
                               pair tiles (default 16 x 16; gradient calls use 8 x 8
                               unless D4_FLAG_ATM16) */
+#define D4_FLAG_ONE_STREAM 2048 /* testing/profiling: run the three-body side on the
+                                   caller's stream too (no companion stream), so a
+                                   kernel trace shows each kernel's own time */
 
 /* Element and reference-system tables, host memory, read once by
  * d4_context_create. Per-element arrays have D4_MAX_Z entries; per-reference

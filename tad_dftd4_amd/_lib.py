@@ -34,6 +34,7 @@ D4_FLAG_ROWS_32 = 512
 D4_FLAG_ROWS_64 = 768
 D4_FLAG_ROWS_MASK = 768
 D4_FLAG_ATM8 = 1024
+D4_FLAG_ONE_STREAM = 2048
 D4_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 SLOTS = ("prep", "cn", "eeq", "weights", "pairs", "atm", "atm_rows", "adjoint", "chain")

@@ -100,6 +100,7 @@ d4::Params to_params(const d4_params* p) {
   const unsigned rows = ((unsigned)p->flags & D4_FLAG_ROWS_MASK) >> 8;
   k.row_lanes = rows == 0 ? 0 : 8 << rows;  // 1, 2, 3 -> 16, 32, 64
   k.atm8 = (p->flags & D4_FLAG_ATM8) ? 1 : 0;
+  k.one_stream = (p->flags & D4_FLAG_ONE_STREAM) ? 1 : 0;
   return k;
 }
 
@@ -295,7 +296,10 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
     D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, energy, grad), "three-body term");
     return D4_OK;
   }
-  D4_TRY(fork_aux(ctx, c), "fork companion stream");
+  if (c.p.one_stream)
+    c.aux.s = c.s;  // the three-body side in order on the caller's stream
+  else
+    D4_TRY(fork_aux(ctx, c), "fork companion stream");
   if (eeq) {
     D4_RUN_AUX(D4_SLOT_WEIGHTS, d4::launch_weights(c.aux.s, ctx->t, c.p, c.w, c.in, d4::kPairsAtm),
                "reference weights (q = 0)");
@@ -317,7 +321,7 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
          "two-body term");
   if (grad)
     D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
-  D4_TRY(join_aux(c), "join companion stream");
+  if (!c.p.one_stream) D4_TRY(join_aux(c), "join companion stream");
   return D4_OK;
 }
 

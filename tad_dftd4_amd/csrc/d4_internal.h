@@ -72,6 +72,7 @@ struct Params {
   int pair_owner;  // large systems: pair-owner three-body kernel (testing)
   int row_lanes;   // lanes per atom row of the row passes (16, 32, 64; 0: by length)
   int atm8;        // energy-only calls: one-pass three-body kernel with 8 x 8 tiles
+  int one_stream;  // no companion stream (testing, profiling)
 };
 
 // Workspace views for one batch (nbatch molecules padded to natoms).

@@ -114,6 +114,7 @@ class KernelParams:
     # gradients). With both at 0 an activated shape table decides.
     row_lanes: int = 0
     atm_tile: int = 0
+    one_stream: bool = False  # profiling: no companion stream for the three-body side
 
     def shapes(self, natoms: int | None = None, grad: bool = False) -> tuple[int, int]:
         """(row_lanes, atm_tile) for a call on a batch padded to ``natoms``."""
@@ -138,6 +139,7 @@ class KernelParams:
             | (_lib.D4_FLAG_PAIR_OWNER if self.pair_owner else 0)
             | _ROW_FLAGS[rows]
             | (_lib.D4_FLAG_ATM8 if tile == 8 else 0)
+            | (_lib.D4_FLAG_ONE_STREAM if self.one_stream else 0)
         )
         return _lib.Params(
             self.s6, self.s8, self.s10, self.s9, self.a1, self.a2, self.alp,

@@ -34,6 +34,7 @@ and the results agree with the request alone to rounding.
 from __future__ import annotations
 
 import collections
+import functools
 import gc
 import time
 from dataclasses import dataclass
@@ -114,6 +115,7 @@ class _Slot:
             self.host = torch.empty(host_bytes, dtype=torch.uint8).pin_memory()
 
 
+@functools.lru_cache(maxsize=8192)
 def _layout(nbatch: int, natoms: int, forces: bool) -> dict:
     """Byte offsets of one batch in a slot: inputs (numbers, positions,
     charge) and outputs (energy, gradient) in both buffers, the workspace on
@@ -143,27 +145,40 @@ def footprint(nbatch: int, natoms: int, forces: bool) -> int:
     return _layout(nbatch, natoms, forces)["dev_end"]
 
 
+def _shape_error(req: Request, slot_budget: int) -> str | None:
+    """The checks that need no pass over the request's values: atom count,
+    array types and shapes, footprint (cached per padded length)."""
+    numbers, positions = req.numbers, req.positions
+    if not isinstance(numbers, np.ndarray) or not isinstance(positions, np.ndarray):
+        return "numbers and positions must be numpy arrays"
+    n = len(numbers)
+    if n < 1:
+        return "a request needs at least one atom"
+    if numbers.ndim != 1 or numbers.dtype.kind not in "iu":
+        return "atomic numbers must be a 1-d integer array"
+    if positions.shape != (n, 3):
+        return f"positions must have shape ({n}, 3), got {positions.shape}"
+    if footprint(1, size_class(n), req.forces) > slot_budget:
+        return f"a {n}-atom request exceeds the per-slot memory budget ({slot_budget} bytes)"
+    return None
+
+
+def _value_error(req: Request) -> str | None:
+    """The checks on the request's values: atomic-number range, finiteness."""
+    if req.numbers.min() < 0 or req.numbers.max() >= 104:
+        return "atomic numbers must lie in 0..103 (0 marks padding)"
+    if not np.isfinite(req.positions).all() or not np.isfinite(req.charge):
+        return "positions and charge must be finite"
+    return None
+
+
 def request_error(req: Request, slot_budget: int) -> str | None:
     """Why ``req`` cannot be served by a slot of ``slot_budget`` device
     bytes (None if it can): no atoms, atomic numbers outside 0..103,
     positions of the wrong shape, non-finite inputs, or a footprint above the
-    budget even alone."""
-    numbers = np.asarray(req.numbers)
-    positions = np.asarray(req.positions)
-    n = len(numbers)
-    if n < 1:
-        return "a request needs at least one atom"
-    if numbers.ndim != 1 or not np.issubdtype(numbers.dtype, np.integer):
-        return "atomic numbers must be a 1-d integer array"
-    if numbers.min() < 0 or numbers.max() >= 104:
-        return "atomic numbers must lie in 0..103 (0 marks padding)"
-    if positions.shape != (n, 3):
-        return f"positions must have shape ({n}, 3), got {positions.shape}"
-    if not np.isfinite(positions).all() or not np.isfinite(req.charge):
-        return "positions and charge must be finite"
-    if footprint(1, size_class(n), req.forces) > slot_budget:
-        return f"a {n}-atom request exceeds the per-slot memory budget ({slot_budget} bytes)"
-    return None
+    budget even alone. (The serving loop makes the shape checks at
+    submission and the value checks once per batch on the packed arrays.)"""
+    return _shape_error(req, slot_budget) or _value_error(req)
 
 
 def _view(buf: torch.Tensor, off: int, dtype, shape) -> torch.Tensor:
@@ -209,13 +224,17 @@ class D4Server:
         """Why ``req`` cannot be served (None if it can)."""
         return request_error(req, self.slot_budget)
 
+    def _fail(self, req: Request, why: str, batch: int = 0) -> None:
+        self.results.append(Result(req.rid, None, None, req.arrival, time.perf_counter(),
+                                   batch, why))
+
     def submit(self, req: Request) -> None:
         """Queue ``req``; a request that cannot be served gets a failed
-        ``Result`` at once and does not enter a batch."""
-        why = self.validate(req)
+        ``Result`` (at once for shape and size problems, when its batch is
+        packed for bad values) and does not enter a batch."""
+        why = _shape_error(req, self.slot_budget)
         if why is not None:
-            self.results.append(Result(req.rid, None, None, req.arrival, time.perf_counter(),
-                                       0, why))
+            self._fail(req, why)
             return
         self.queues[(req.forces, size_class(len(req.numbers)))].append(req)
 
@@ -238,25 +257,45 @@ class D4Server:
             raise MemoryError(f"a {npad}-atom request exceeds the per-slot memory budget "
                               f"({self.slot_budget} bytes)")
         reqs = [q.popleft() for _ in range(count)]
+        zc = np.concatenate([r.numbers for r in reqs])
+        pc = np.concatenate([r.positions for r in reqs])
+        charges = np.fromiter((r.charge for r in reqs), dtype=np.float64, count=count)
+        if (zc.min() < 0 or zc.max() >= 104 or not np.isfinite(pc).all()
+                or not np.isfinite(charges).all()):
+            # rare: find the bad requests, fail them alone, pack the rest
+            good = []
+            for r in reqs:
+                why = _value_error(r)
+                if why is None:
+                    good.append(r)
+                else:
+                    self._fail(r, why)
+            reqs = good
+            count = len(reqs)
+            if count == 0:
+                return False
+            zc = np.concatenate([r.numbers for r in reqs])
+            pc = np.concatenate([r.positions for r in reqs])
+            charges = np.fromiter((r.charge for r in reqs), dtype=np.float64, count=count)
         slot = self._free.pop()
         lay = _layout(count, npad, forces)
         slot.reserve(lay["dev_end"], lay["host_end"])
         # pack into the pinned buffer: zero padding, then scatter every
         # request's atoms to (row, column) in one vectorised copy
         lens = np.fromiter((len(r.numbers) for r in reqs), dtype=np.int64, count=count)
-        rows = np.repeat(np.arange(count), lens)
-        cols = np.arange(int(lens.sum())) - np.repeat(np.cumsum(lens) - lens, lens)
+        # flat destination of every atom: row k starts at k * npad
+        start = np.arange(0, count * npad, npad, dtype=np.int64)
+        flat = np.arange(int(lens.sum()), dtype=np.int64) + np.repeat(
+            start - (np.cumsum(lens) - lens), lens)
         h = slot.host.numpy()
         hn = h[lay["numbers"]: lay["numbers"] + 8 * count * npad].view(np.int64)
-        hn = hn.reshape(count, npad)
         hp = h[lay["positions"]: lay["positions"] + 24 * count * npad].view(np.float64)
-        hp = hp.reshape(count, npad, 3)
         hc = h[lay["charge"]: lay["charge"] + 8 * count].view(np.float64)
-        hn.fill(0)
+        hn.fill(0)  # padding atoms: Z = 0 at the origin
         hp.fill(0.0)
-        hn[rows, cols] = np.concatenate([r.numbers for r in reqs])
-        hp[rows, cols] = np.concatenate([r.positions for r in reqs])
-        hc[:] = [r.charge for r in reqs]
+        hn[flat] = zc
+        hp.reshape(-1, 3)[flat] = pc
+        hc[:] = charges
         with torch.cuda.stream(slot.stream):
             slot.dev[: lay["in_end"]].copy_(slot.host[: lay["in_end"]], non_blocking=True)
             nt = _view(slot.dev, lay["numbers"], torch.int64, (count, npad))
@@ -302,24 +341,32 @@ class D4Server:
             if forces:
                 g = h[lay["grad"]: lay["grad"] + 24 * count * npad].view(np.float64)
                 g = g.reshape(count, npad, 3)
-            status = h[lay["status"]: lay["status"] + 4 * count].view(np.int32)
+            status = h[lay["status"]: lay["status"] + 4 * count].view(np.int32).copy()
+            # one copy of the batch's outputs out of the pinned buffer (reused by
+            # the slot's next batch); each result is a view of its rows
+            e = e.copy()
+            g = None if g is None else g.copy()
+            bad = status & (STATUS_SINGULAR | STATUS_BAD_Z)
+            finite = np.isfinite(e).all(axis=1)
+            if g is not None:
+                finite &= np.isfinite(g).all(axis=(1, 2))
+            ok = (bad == 0) & finite
+            results = self.results
+            bid = slot.bid
             for k, r in enumerate(slot.reqs):
                 n = len(r.numbers)
-                ek = e[k, :n]
-                gk = None if g is None else g[k, :n]
-                why = None
+                if ok[k]:
+                    results.append(Result(r.rid, e[k, :n], None if g is None else g[k, :n],
+                                          r.arrival, now, bid))
+                    continue
                 if status[k] & STATUS_SINGULAR:
                     why = "the EEQ system of this molecule is singular"
                 elif status[k] & STATUS_BAD_Z:
                     why = "atomic numbers out of range"
-                elif not np.isfinite(ek).all() or (gk is not None and not np.isfinite(gk).all()):
-                    why = "non-finite result"
-                if why is None:
-                    self.results.append(Result(r.rid, ek.copy(), None if gk is None else gk.copy(),
-                                               r.arrival, now, slot.bid))
                 else:
-                    self.results.append(Result(r.rid, None, None, r.arrival, now, slot.bid, why))
-                added += 1
+                    why = "non-finite result"
+                results.append(Result(r.rid, None, None, r.arrival, now, bid, why))
+            added += len(slot.reqs)
             slot.reqs = None
             self._free.append(slot)
         self.inflight = still

@@ -103,11 +103,15 @@ def test_two_streams_share_a_context():
         assert torch.equal(e1, ea) and torch.equal(g1, ga) and torch.equal(e2, eb)
 
 
-def test_bad_requests_fail_alone():
+def test_bad_requests_fail_alone(monkeypatch):
     """One request per failure kind in a stream of good ones: only those
-    fail. Two are rejected at submission (atomic number 120, a 3000-atom
-    request above a slot's budget); one with Z = 110 is queued past the
-    submission check, so the device's status word flags it inside its batch."""
+    fail. A 3000-atom request above a slot's budget is rejected at
+    submission; atomic number 120 and a non-finite position are caught when
+    their batch is packed; a request with Z = 110 that gets past the host
+    checks (switched off for it here) is flagged by the device's status word
+    inside its batch."""
+    from tad_dftd4_amd import serve
+
     print("[case] serving stream with bad requests", flush=True)
     cut = Cutoff(disp2=50.0, disp3=25.0, cn_eeq=25.0)
     server = D4Server(PARAM, cut, memory_budget=256 << 20, max_inflight=2, max_batch=16)
@@ -117,19 +121,25 @@ def test_bad_requests_fail_alone():
     rng = np.random.default_rng(2)
     huge = synth.molecule(rng, 3000, synth.MIX_HCNO, 2.8, (0,))
     too_big = Request(1001, huge.numbers, huge.positions, 0.0, forces=True)
-    planted = synth.molecule(rng, 20, synth.MIX_HCNO, 2.8, (0,))
-    planted.numbers[5] = 110
+    nan_pos = synth.molecule(rng, 12, synth.MIX_HCNO, 2.8, (0,))
+    nan_pos.positions[3, 1] = np.nan
+    bad_pos = Request(1003, nan_pos.numbers, nan_pos.positions, 0.0, forces=True)
     arrivals = np.cumsum(np.random.default_rng(4).exponential(2e-4, size=len(reqs)))
     server.submit(bad_z)
     server.submit(too_big)
-    server.queues[(False, 32)].append(Request(1002, planted.numbers, planted.positions, 0.0,
-                                              forces=False))
+    server.submit(bad_pos)
     results = server.run_stream(reqs, arrivals)
+    planted = synth.molecule(rng, 20, synth.MIX_HCNO, 2.8, (0,))
+    planted.numbers[5] = 110
+    monkeypatch.setattr(serve, "_value_error", lambda r: None)
+    server.submit(Request(1002, planted.numbers, planted.positions, 0.0, forces=False))
+    server.drain()
     by_id = {r.rid: r for r in server.results}
-    assert "0..103" in by_id[1000].error
     assert "memory budget" in by_id[1001].error
+    assert "0..103" in by_id[1000].error
+    assert "finite" in by_id[1003].error
     assert "out of range" in by_id[1002].error
-    for r in (1000, 1001, 1002):
+    for r in (1000, 1001, 1002, 1003):
         assert by_id[r].energy is None and by_id[r].gradient is None
     good = [r for r in results if r.rid < 1000]
     assert len(good) == len(reqs) and all(r.error is None for r in good)
@@ -143,4 +153,4 @@ def test_bad_requests_fail_alone():
         else:
             e, _ = runtime.forward(numbers, positions, charge, kp)
         assert np.array_equal(by_id[req.rid].energy, e[0].cpu().numpy())
-    assert latency_stats(server.results)["failed"] == 3
+    assert latency_stats(server.results)["failed"] == 4
