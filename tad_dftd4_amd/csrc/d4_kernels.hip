@@ -1347,11 +1347,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
 // partition as k_atm1; the smaller tiles waste fewer slots on the diagonal
 // tiles, on third atoms z <= y and past the end of the molecule.
 //
-// Work-group (m, s) takes every S-th pair tile (bx <= by) of 8-atom blocks of
-// molecule m, S = atm8_splits(n). Lane (tx, ty) = (lane & 7, lane >> 3) owns
-// the pair x = 8 bx + tx < y = 8 by + ty and visits the third atoms z > y in
-// chunks of 8. Inside a chunk lane tx of a row handles z = z0 + (tx ^ k) at
-// step k, and the running sums for z move to the lane that handles z at the
+// Every triple of the 8-atom blocks of molecule m is counted once, by one of
+// two kinds of tiles, and work-group (m, s) takes every S-th tile (S =
+// atm8_splits(n)):
+//  - W-tile t (the first ceil(nb / 2) tiles): the 28 pairs u < v inside
+//    block 2 t (lanes 0..27) and inside block 2 t + 1 (lanes 32..59) against
+//    the third atoms of every block; in the pair's own block only z > v
+//    counts. This takes the triples with two or three atoms in one block.
+//  - pair tile (bx, by), bx < by <= nb - 2: lane (tx, ty) = (lane & 7,
+//    lane >> 3) owns x = 8 bx + tx, y = 8 by + ty and takes the third atoms
+//    of the blocks above by (three distinct blocks).
+// A triple's energy and gradient are symmetric in which two atoms form the
+// pair, so the roles follow the tile. Compared with tiles bx <= by over the
+// third atoms z > y, this drops the half-empty diagonal tiles and the
+// masked first chunks: 19% fewer lane-slots on a 1024 x 20-120 batch.
+// The third atoms come in chunks of 8. Inside a chunk lane tx of a row (lanes
+// 8 r .. 8 r + 7) handles z = z0 + (tx ^ k) at step k, and the running sums
+// for z move to the lane that handles z at the
 // next step: lane L takes the sums of lane L ^ (k ^ (k + 1)), a swap inside
 // each quad (xor 1 or 3) or a mirror of the 8-lane half row (xor 7), single
 // DPP moves; one more xor 7 after the last step (so the moves repeat every
@@ -1529,7 +1541,8 @@ __device__ __forceinline__ void atm8_stage(const Atm8Chunk& c, double* s_zc, dou
 template <bool ENERGY, bool GRAD, bool PG, bool FULL, bool FIRST>
 __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, Atm8Sums& u,
                                            Atm8Ring& rr, const double* s_zc, const double* s_r,
-                                           const double* s_rt, int tx, int ty, int z0, int y) {
+                                           const double* s_rt, int tx, int ty, int z0, int y,
+                                           int cx = -1, int cy = -1, bool own = true) {
   constexpr int T8 = kAtm8;
   const double c3sq = p.cut3sq, alp = p.alp, a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
   // four steps per iteration (two with parameter gradients, whose extra
@@ -1545,8 +1558,9 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
     const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
     const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
     const double* zrow = s_zc + __umul24(zl, kAtm8Row);
-    const double2* px = reinterpret_cast<const double2*>(zrow + 4 * tx);
-    const double2* py = reinterpret_cast<const double2*>(zrow + 4 * (T8 + ty));
+    // staged record columns of x and y: the tile's x block, then its y block
+    const double2* px = reinterpret_cast<const double2*>(zrow + 4 * (cx < 0 ? tx : cx));
+    const double2* py = reinterpret_cast<const double2*>(zrow + 4 * (cy < 0 ? T8 + ty : cy));
     const double2 x01 = px[0], y01 = py[0];
     const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
     const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
@@ -1557,8 +1571,9 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
     double P = q.pxy * x01.y * y01.y;                // s9 is in pxy
     double P0 = PG ? q.pxy0 * x01.y * y01.y : 0.0;   // without s9
     if (FIRST) {
-      // third atoms up to y carry no weight (z may coincide with x or y)
-      const bool on = z0 + zl > y;
+      // third atoms up to y carry no weight (z may coincide with x or y);
+      // `own`: the chunk is y's block for this lane
+      const bool on = !own || z0 + zl > y;
       bb = on ? bb : 1.0;
       cc = on ? cc : 1.0;
       P = on ? P : 0.0;
@@ -1700,6 +1715,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
   __shared__ __align__(16) double s_r[T8 * 6];  // z position, dL/dE_z
   __shared__ double s_rt[T8];                   // sqrt(r4r2_z) (parameter gradients)
   __shared__ double s_red[NA][64];
+  __shared__ double s_red2[NA][64];
   extern __shared__ double s_acc[];             // [NA][N] atom sums of this split
 
   // block b runs on XCD b mod 8. Molecules go to the XCDs in runs of eight
@@ -1717,7 +1733,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
   const int Sm = atm8_splits(n);  // this molecule's split count (S is the batch maximum)
   if (s >= Sm) return;
   const int nb = (n + T8 - 1) / T8;
-  const int tiles = nb * (nb + 1) / 2;
+  const int nw = (nb + 1) / 2;  // W-tiles, then the pair tiles bx < by <= nb - 2
+  const int tiles = nw + (nb > 1 ? (nb - 1) * (nb - 2) / 2 : 0);
   const int lane = threadIdx.x, tx = lane & (T8 - 1), ty = lane >> 3;
   const int64_t* Z = in.numbers + m * N;
   const double* R = in.positions + m * N * 3;
@@ -1743,20 +1760,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
     full = d0 * d0 + d1 * d1 + d2 * d2 <= c3sq;
   }
 
+  // W-tile lane roles: lane (h, p) = (lane >> 5, lane & 31) owns the pair
+  // p = v (v - 1) / 2 + u (u < v) of block 2 t + h; lanes p >= 28 are idle
+  // (they read the records of the pair (0, 1), with zero pair factors)
+  const int wh = lane >> 5, wp = lane & 31;
+  const bool wok = wp < 28;
+  int wv = 1;
+  while (wok && (wv + 1) * wv / 2 <= wp) ++wv;
+  const int wu = wok ? wp - wv * (wv - 1) / 2 : 0;
+
   for (int t = s; t < tiles; t += Sm) {
-    int by = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while (by * (by + 1) / 2 > t) --by;
-    while ((by + 1) * (by + 2) / 2 <= t) ++by;
-    const int bx = t - by * (by + 1) / 2;
-    const int x = bx * T8 + tx, y = by * T8 + ty;
-    const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
+    const bool wt = t < nw;
+    int bx, by, x, y, cx, cy, z1;  // staged blocks, pair, its record columns, first chunk
+    if (wt) {
+      bx = 2 * t;
+      by = bx + 1;
+      x = T8 * (bx + wh) + wu;
+      y = T8 * (bx + wh) + wv;
+      cx = T8 * wh + wu;
+      cy = T8 * wh + wv;
+      z1 = 0;
+    } else {
+      const int t2 = t - nw;  // = by (by - 1) / 2 + bx
+      by = 1;
+      while ((by + 1) * by / 2 <= t2) ++by;
+      bx = t2 - by * (by - 1) / 2;
+      x = bx * T8 + tx;
+      y = by * T8 + ty;
+      cx = tx;
+      cy = T8 + ty;
+      z1 = (by + 1) * T8;
+    }
+    const bool valid = (wok || !wt) && x < y && y < n && Z[x] != 0 && Z[y] != 0;
     const Atm8Pair q = atm8_pair<GRAD, PG>(p, in, w, m, x, y, valid);
     u.ex = u.ey = u.cx = u.cy = u.sca = 0.0;
     u.fx0 = u.fx1 = u.fx2 = u.fy0 = u.fy1 = u.fy2 = 0.0;
 
     Atm8Chunk ch;
-    atm8_fetch<GRAD, PG>(ch, in, w, m, n, by * T8, bx * T8, by * T8, lane);
-    for (int z0 = by * T8; z0 < n; z0 += T8) {
+    atm8_fetch<GRAD, PG>(ch, in, w, m, n, z1, bx * T8, by * T8, lane);
+    for (int z0 = z1; z0 < n; z0 += T8) {
       __syncthreads();
       atm8_stage<PG>(ch, s_zc, s_r, s_rt, lane);
       __syncthreads();
@@ -1764,18 +1806,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
         atm8_fetch<GRAD, PG>(ch, in, w, m, n, z0 + T8, bx * T8, by * T8, lane);
 
       Atm8Ring rr = {0.0, 0.0, 0.0, 0.0, 0.0};
-      const bool first = z0 == by * T8;
+      // the two chunks of a W-tile's own blocks: third atoms up to y of the
+      // lane's own block carry no weight
+      const bool first = wt && (z0 >> 4) == t;
+      const bool own = z0 == T8 * (bx + wh);
       if (full) {
         if (first)
-          atm8_chunk<ENERGY, GRAD, PG, true, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y);
+          atm8_chunk<ENERGY, GRAD, PG, true, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y,
+                                                   cx, cy, own);
         else
-          atm8_chunk<ENERGY, GRAD, PG, true, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y);
+          atm8_chunk<ENERGY, GRAD, PG, true, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y,
+                                                    cx, cy);
       } else {
         if (first)
-          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y);
+          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y,
+                                                    cx, cy, own);
         else
           atm8_chunk<ENERGY, GRAD, PG, false, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0,
-                                                     y);
+                                                     y, cx, cy);
       }
       if (full) rr.E *= 2.0;
       // lane tx of every row now holds that row's sums for z0 + tx (the
@@ -1792,7 +1840,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
         }
       }
     }
-    // tile sums of the x atoms (over the rows) and the y atoms (over the columns)
+    // tile sums of the x and the y atoms
     if (full) {
       u.ex *= 2.0;
       u.ey = u.ex;
@@ -1800,24 +1848,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
     atm8_fold_sca(q, u);
     __syncthreads();
     atm8_stage_sums<NA>(s_red, lane, u.ex, u.fx0, u.fx1, u.fx2, u.cx);
+    atm8_stage_sums<NA>(s_red2, lane, u.ey, u.fy0, u.fy1, u.fy2, u.cy);
     __syncthreads();
-    if (lane < T8 * NA) {
-      const int col = lane & (T8 - 1), c = lane >> 3;
-      if ((c == 0 ? ENERGY : GRAD) && bx * T8 + col < n) {
-        double sum = 0.0;
-        for (int r = 0; r < T8; ++r) sum += s_red[c][r * T8 + col];
-        s_acc[c * n + bx * T8 + col] += sum;
+    if (wt) {
+      // atom a of block 2 t + hh: x of the pairs (a, v > a), y of (u < a, a)
+      for (int e = lane; e < 2 * T8 * NA; e += 64) {
+        const int c = e >> 4, a16 = e & 15, hh = a16 >> 3, a = a16 & 7;
+        const int atom = 2 * T8 * t + a16;
+        if ((c == 0 ? ENERGY : GRAD) && atom < n) {
+          const double* sx = s_red[c] + 32 * hh;
+          const double* sy = s_red2[c] + 32 * hh + a * (a - 1) / 2;
+          double sum = 0.0;
+          for (int v = a + 1; v < T8; ++v) sum += sx[v * (v - 1) / 2 + a];
+          for (int u0 = 0; u0 < a; ++u0) sum += sy[u0];
+          s_acc[c * n + atom] += sum;
+        }
       }
-    }
-    __syncthreads();
-    atm8_stage_sums<NA>(s_red, lane, u.ey, u.fy0, u.fy1, u.fy2, u.cy);
-    __syncthreads();
-    if (lane < T8 * NA) {
-      const int row = lane & (T8 - 1), c = lane >> 3;
-      if ((c == 0 ? ENERGY : GRAD) && by * T8 + row < n) {
+    } else if (lane < T8 * NA) {
+      // x atoms over the rows, y atoms over the columns
+      const int l = lane & (T8 - 1), c = lane >> 3;
+      if ((c == 0 ? ENERGY : GRAD) && bx * T8 + l < n) {
         double sum = 0.0;
-        for (int q2 = 0; q2 < T8; ++q2) sum += s_red[c][row * T8 + q2];
-        s_acc[c * n + by * T8 + row] += sum;
+        for (int r = 0; r < T8; ++r) sum += s_red[c][r * T8 + l];
+        s_acc[c * n + bx * T8 + l] += sum;
+      }
+      if ((c == 0 ? ENERGY : GRAD) && by * T8 + l < n) {
+        double sum = 0.0;
+        for (int q2 = 0; q2 < T8; ++q2) sum += s_red2[c][l * T8 + q2];
+        s_acc[c * n + by * T8 + l] += sum;
       }
     }
   }
