@@ -567,7 +567,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const double* __restrict__ gw0 = w.gw0;
   const int zi = i < n ? (int)Z[i] : 0;
   if (zi == 0) {
-    if (ATMF) {
+    // rows past the molecule are never read (readers stop at row n)
+    if (ATMF && i < n) {
       for (int j = lane; j < n; j += RL) {
         ptr[j] = 0.0;
         ppr[j] = 0.0;
@@ -1395,10 +1396,23 @@ __device__ __forceinline__ void ring_xor8(Atm8Ring& r, bool energy, bool grad) {
 
 
 constexpr double kAtmFar = 1.0e6;  // position of lanes and third atoms without an atom (bohr)
-// [z][x block | y block] pair records (t, p, kz, kc) of a chunk; the row
-// stride of 33 16-byte units (odd) keeps the skewed reads of a row on
-// distinct banks
-constexpr int kAtm8Row = 66;
+// Staged chunk: row zl (1024 bytes) of third atom z0 + zl holds 32-byte
+// records: c = 0..15 the pair records (t, p, kz, kc) against the x block
+// (c < 8) and the y block, c = 16 the third atom's position and dL/dE, c = 17
+// its sqrt(r4r2) (parameter gradients). Record c of row zl sits at byte
+// (zl << 10) | ((c << 5) ^ (zl << 4)): the row bits and the record bits do
+// not overlap, so with zl = tx ^ k the address is a lane constant xor a step
+// constant (one v_xor per read, no multiply), and the (zl << 4) skew puts
+// the eight lanes of a row on distinct 16-byte bank groups.
+constexpr int kAtm8Row = 128;  // doubles per row
+constexpr int kAtm8RecZ = 16, kAtm8RecRt = 17;
+__device__ __forceinline__ uint32_t atm8_rec(int zl, int c) {
+  return ((uint32_t)zl << 10) | (((uint32_t)c << 5) ^ ((uint32_t)zl << 4));
+}
+template <typename T>
+__device__ __forceinline__ const T* atm8_at(const double* s_zc, uint32_t off) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(s_zc) + off);
+}
 
 // The pair (x, y) a lane owns. Lanes without a pair sit far from every atom
 // (no zero distance) with zero pair factors.
@@ -1464,6 +1478,7 @@ __device__ __forceinline__ Atm8Pair atm8_pair(const Params& p, const Inputs& in,
 // (the (x - y) part as the scalar sca) and the parameter gradients (x 6).
 struct Atm8Sums {
   double ex, ey, cx, cy, sca, fx0, fx1, fx2, fy0, fy1, fy2;
+  double sal, sw;  // per tile: sums of al (x 1 / r_xy^2 into sca) and of We (x kxy, kyx)
   double g_s9, g_alp, g_a1, g_a2;
 };
 
@@ -1513,23 +1528,21 @@ __device__ __forceinline__ void atm8_fetch(Atm8Chunk& c, const Inputs& in, const
 }
 
 template <bool PG>
-__device__ __forceinline__ void atm8_stage(const Atm8Chunk& c, double* s_zc, double* s_r,
-                                           double* s_rt, int lane) {
+__device__ __forceinline__ void atm8_stage(const Atm8Chunk& c, double* s_zc, int lane) {
+  char* base = reinterpret_cast<char*>(s_zc);
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int e = lane + 64 * r;
     const int zl = e >> 4, cl = e & 15;
-    double2* dst = reinterpret_cast<double2*>(s_zc + zl * kAtm8Row + 4 * cl);
+    double2* dst = reinterpret_cast<double2*>(base + atm8_rec(zl, cl));
     dst[0] = make_double2(c.t[r], c.p[r]);
     dst[1] = make_double2(c.kz[r], c.kc[r]);
   }
   if (lane < kAtm8) {
-    double* r = s_r + 6 * lane;
-    r[0] = c.r[0];
-    r[1] = c.r[1];
-    r[2] = c.r[2];
-    r[3] = c.r[3];
-    if (PG) s_rt[lane] = c.r[4];
+    double2* r = reinterpret_cast<double2*>(base + atm8_rec(lane, kAtm8RecZ));
+    r[0] = make_double2(c.r[0], c.r[1]);
+    r[1] = make_double2(c.r[2], c.r[3]);
+    if (PG) *reinterpret_cast<double*>(base + atm8_rec(lane, kAtm8RecRt)) = c.r[4];
   }
 }
 
@@ -1540,10 +1553,16 @@ __device__ __forceinline__ void atm8_stage(const Atm8Chunk& c, double* s_zc, dou
 // tx of every row holds that row's ring sums for z0 + tx.
 template <bool ENERGY, bool GRAD, bool PG, bool FULL, bool FIRST>
 __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, Atm8Sums& u,
-                                           Atm8Ring& rr, const double* s_zc, const double* s_r,
-                                           const double* s_rt, int tx, int ty, int z0, int y,
-                                           int cx = -1, int cy = -1, bool own = true) {
+                                           Atm8Ring& rr, const double* s_zc, int tx, int ty,
+                                           int z0, int y, int cx = -1, int cy = -1,
+                                           bool own = true) {
   constexpr int T8 = kAtm8;
+  // lane parts of the record addresses (the step part is (k << 10) | (k << 4))
+  const uint32_t lrow = ((uint32_t)tx << 10) | ((uint32_t)tx << 4);
+  const uint32_t ox = lrow ^ ((uint32_t)(cx < 0 ? tx : cx) << 5);
+  const uint32_t oy = lrow ^ ((uint32_t)(cy < 0 ? T8 + ty : cy) << 5);
+  const uint32_t oz = lrow ^ ((uint32_t)kAtm8RecZ << 5);
+  const uint32_t ort = lrow ^ ((uint32_t)kAtm8RecRt << 5);
   const double c3sq = p.cut3sq, alp = p.alp, a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
   // four steps per iteration (two with parameter gradients, whose extra
   // sums would not fit the registers), so every ring move below is static
@@ -1553,14 +1572,13 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
   constexpr int kUnroll = PG ? 2 : D4_ATM8_UNROLL;
   D4_UNROLL(kUnroll)
   for (int k = 0; k < T8; ++k) {
-    const int zl = tx ^ k;
-    const double2 rz01 = *reinterpret_cast<const double2*>(s_r + 6 * zl);
-    const double2 rz23 = *reinterpret_cast<const double2*>(s_r + 6 * zl + 2);
+    const uint32_t ks = ((uint32_t)k << 10) | ((uint32_t)k << 4);
+    const double2* pz = atm8_at<double2>(s_zc, oz ^ ks);
+    const double2 rz01 = pz[0], rz23 = pz[1];
     const double4 rz = make_double4(rz01.x, rz01.y, rz23.x, rz23.y);
-    const double* zrow = s_zc + __umul24(zl, kAtm8Row);
-    // staged record columns of x and y: the tile's x block, then its y block
-    const double2* px = reinterpret_cast<const double2*>(zrow + 4 * (cx < 0 ? tx : cx));
-    const double2* py = reinterpret_cast<const double2*>(zrow + 4 * (cy < 0 ? T8 + ty : cy));
+    // records of x and y against z = z0 + (tx ^ k)
+    const double2* px = atm8_at<double2>(s_zc, ox ^ ks);
+    const double2* py = atm8_at<double2>(s_zc, oy ^ ks);
     const double2 x01 = px[0], y01 = py[0];
     const double2 x23 = GRAD ? px[1] : make_double2(0.0, 0.0);
     const double2 y23 = GRAD ? py[1] : make_double2(0.0, 0.0);
@@ -1573,7 +1591,7 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
     if (FIRST) {
       // third atoms up to y carry no weight (z may coincide with x or y);
       // `own`: the chunk is y's block for this lane
-      const bool on = !own || z0 + zl > y;
+      const bool on = !own || z0 + (tx ^ k) > y;
       bb = on ? bb : 1.0;
       cc = on ? cc : 1.0;
       P = on ? P : 0.0;
@@ -1628,12 +1646,12 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
       const double KW = K * W6;  // half of the dL/de-weighted factor; x2 at the end
       const double al = KW * H, be = KW * G2;
       const double ali = al * a * iabc;  // al / (b c)
-      const double ca = fma(al, q.ia, be * dsa);
       const double cb = fma(ali, cc, be * dsb);
       const double cg = fma(ali, bb, be * dsc);
       // x gets ca (x - y) + cb (x - z), y gets cg (y - z) - ca (x - y):
       // the (x - y) part is summed as a scalar
-      u.sca += ca;
+      u.sal += al;  // ca = al / a + be dsa, the 1 / a applied per tile
+      u.sca = fma(be, dsa, u.sca);
       u.fx0 = fma(cb, u0, u.fx0);
       u.fx1 = fma(cb, u1, u.fx1);
       u.fx2 = fma(cb, u2, u.fx2);
@@ -1643,14 +1661,16 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
       rr.F0 = fma(-cb, u0, fma(-cg, v0, rr.F0));
       rr.F1 = fma(-cb, u1, fma(-cg, v1, rr.F1));
       rr.F2 = fma(-cb, u2, fma(-cg, v2, rr.F2));
-      u.cx = fma(We, q.kxy + x23.y, u.cx);
-      u.cy = fma(We, q.kyx + y23.y, u.cy);
+      u.sw += We;  // times kxy and kyx per tile
+      u.cx = fma(We, x23.y, u.cx);
+      u.cy = fma(We, y23.y, u.cy);
       rr.D = fma(We, x23.x + y23.x, rr.D);
       if (PG) {
         const double TdT = -6.0 * f * We * T;  // dL6/dT times T
         u.g_s9 = fma(W6, P0 * f * g, u.g_s9);
         u.g_alp = fma(TdT, T > 0.0 ? log(T) / alp : 0.0, u.g_alp);
-        const double sxz = sq3 * q.rx4 * s_rt[zl], syz = sq3 * q.ry4 * s_rt[zl];
+        const double rtz = *atm8_at<double>(s_zc, ort ^ ks);
+        const double sxz = sq3 * q.rx4 * rtz, syz = sq3 * q.ry4 * rtz;
         const double bxz = rcp_nt(fma(a1p, sxz, a2p)), byz = rcp_nt(fma(a1p, syz, a2p));
         u.g_a1 = fma(TdT * alp / 3.0, q.Axy + sxz * bxz + syz * byz, u.g_a1);
         u.g_a2 = fma(TdT * alp / 3.0, q.Bxy + bxz + byz, u.g_a2);
@@ -1671,13 +1691,16 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
 // Fold the (x - y) force scalar of a tile into the x and y gradient sums.
 __device__ __forceinline__ void atm8_fold_sca(const Atm8Pair& q, Atm8Sums& u) {
   const double d0 = q.rx0 - q.ry0, d1 = q.rx1 - q.ry1, d2 = q.rx2 - q.ry2;
-  u.fx0 = fma(u.sca, d0, u.fx0);
-  u.fx1 = fma(u.sca, d1, u.fx1);
-  u.fx2 = fma(u.sca, d2, u.fx2);
-  u.fy0 = fma(-u.sca, d0, u.fy0);
-  u.fy1 = fma(-u.sca, d1, u.fy1);
-  u.fy2 = fma(-u.sca, d2, u.fy2);
-  u.sca = 0.0;
+  const double sca = fma(u.sal, q.ia, u.sca);
+  u.fx0 = fma(sca, d0, u.fx0);
+  u.fx1 = fma(sca, d1, u.fx1);
+  u.fx2 = fma(sca, d2, u.fx2);
+  u.fy0 = fma(-sca, d0, u.fy0);
+  u.fy1 = fma(-sca, d1, u.fy1);
+  u.fy2 = fma(-sca, d2, u.fy2);
+  u.cx = fma(q.kxy, u.sw, u.cx);
+  u.cy = fma(q.kyx, u.sw, u.cy);
+  u.sca = u.sal = u.sw = 0.0;
 }
 
 // Stage five per-lane values (energy share, gradient at half weight, dL/dCN)
@@ -1712,8 +1735,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
   constexpr int NA = kAtm1Acc;
   constexpr int T8 = kAtm8;
   __shared__ __align__(16) double s_zc[T8 * kAtm8Row];
-  __shared__ __align__(16) double s_r[T8 * 6];  // z position, dL/dE_z
-  __shared__ double s_rt[T8];                   // sqrt(r4r2_z) (parameter gradients)
   __shared__ double s_red[NA][64];
   __shared__ double s_red2[NA][64];
   extern __shared__ double s_acc[];             // [NA][N] atom sums of this split
@@ -1800,7 +1821,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
     atm8_fetch<GRAD, PG>(ch, in, w, m, n, z1, bx * T8, by * T8, lane);
     for (int z0 = z1; z0 < n; z0 += T8) {
       __syncthreads();
-      atm8_stage<PG>(ch, s_zc, s_r, s_rt, lane);
+      atm8_stage<PG>(ch, s_zc, lane);
       __syncthreads();
       if (z0 + T8 < n)  // in flight during this chunk
         atm8_fetch<GRAD, PG>(ch, in, w, m, n, z0 + T8, bx * T8, by * T8, lane);
@@ -1812,17 +1833,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
       const bool own = z0 == T8 * (bx + wh);
       if (full) {
         if (first)
-          atm8_chunk<ENERGY, GRAD, PG, true, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y,
+          atm8_chunk<ENERGY, GRAD, PG, true, true>(p, q, u, rr, s_zc, tx, ty, z0, y,
                                                    cx, cy, own);
         else
-          atm8_chunk<ENERGY, GRAD, PG, true, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y,
+          atm8_chunk<ENERGY, GRAD, PG, true, false>(p, q, u, rr, s_zc, tx, ty, z0, y,
                                                     cx, cy);
       } else {
         if (first)
-          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0, y,
+          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, tx, ty, z0, y,
                                                     cx, cy, own);
         else
-          atm8_chunk<ENERGY, GRAD, PG, false, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty, z0,
+          atm8_chunk<ENERGY, GRAD, PG, false, false>(p, q, u, rr, s_zc, tx, ty, z0,
                                                      y, cx, cy);
       }
       if (full) rr.E *= 2.0;
@@ -1942,8 +1963,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   constexpr int KS = kAtm8sMaxSplits;
   constexpr int kWin = NA * T8;  // doubles per slot: [NA][8 atoms]
   __shared__ __align__(16) double s_zc[T8 * kAtm8Row];
-  __shared__ __align__(16) double s_r[T8 * 6];
-  __shared__ double s_rt[T8];
   __shared__ double s_red[NA][64];
 
   const int64_t b = blockIdx.x;
@@ -2026,7 +2045,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
       if (gz < G) atm8_fetch<GRAD, PG>(ch, in, w, m, n, gz * T8, gx * T8, gy * T8, lane);
       while (gz < G) {
         __syncthreads();
-        atm8_stage<PG>(ch, s_zc, s_r, s_rt, lane);
+        atm8_stage<PG>(ch, s_zc, lane);
         __syncthreads();
         const int gzn = next_z();
         if (gzn < G)  // in flight during this chunk
@@ -2037,10 +2056,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
         const double old = lane < kWin ? wz[widx] : 0.0;
         Atm8Ring rr = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (gz == gy)
-          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty,
+          atm8_chunk<ENERGY, GRAD, PG, false, true>(p, q, u, rr, s_zc, tx, ty,
                                                     gz * T8, y);
         else
-          atm8_chunk<ENERGY, GRAD, PG, false, false>(p, q, u, rr, s_zc, s_r, s_rt, tx, ty,
+          atm8_chunk<ENERGY, GRAD, PG, false, false>(p, q, u, rr, s_zc, tx, ty,
                                                      gz * T8, y);
         atm8_stage_sums<NA>(s_red, lane, rr.E, rr.F0, rr.F1, rr.F2, rr.D);
         __syncthreads();
