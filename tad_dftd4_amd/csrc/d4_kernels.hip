@@ -552,7 +552,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     DevTables t, Params p, Inputs in, Work w, double* grad_out) {
   constexpr bool ATMF = PART != kPartE2, E2 = PART != kPartAtm;
   const int lane = threadIdx.x & (RL - 1);
-  const int64_t row = (int64_t)blockIdx.x * (256 / RL) + threadIdx.x / RL;
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x / RL) + threadIdx.x / RL;
   if (row >= w.B * w.N) return;
   const int64_t N = w.N;
   const int64_t m = row / N;
@@ -2366,24 +2366,26 @@ __global__ __launch_bounds__(256) void k_pgrad_reduce(Inputs in, Work w, int mod
   }
 }
 
-inline unsigned rows_grid_rl(const Work& w, int rl) {
-  const int per = 256 / rl;
+inline unsigned rows_grid_rl(const Work& w, int rl, int block = 256) {
+  const int per = block / rl;
   return (unsigned)((w.B * w.N + per - 1) / per);
 }
 
 // Launch a row kernel instantiated for the row width of this batch (the
-// three instantiations are passed in parentheses).
-#define D4_LAUNCH_ROWS(K16, K32, K64, s, p, w, ...)               \
-  do {                                                           \
+// three instantiations are passed in parentheses), BLK threads per work-group.
+#define D4_LAUNCH_ROWS_B(BLK, K16, K32, K64, s, p, w, ...)          \
+  do {                                                              \
     const int rl_ = (p).row_lanes ? (p).row_lanes : row_lanes((w).N); \
-    const dim3 g_(rows_grid_rl((w), rl_));                       \
-    if (rl_ == 16)                                               \
-      hipLaunchKernelGGL(K16, g_, dim3(256), 0, s, __VA_ARGS__); \
-    else if (rl_ == 32)                                          \
-      hipLaunchKernelGGL(K32, g_, dim3(256), 0, s, __VA_ARGS__); \
-    else                                                         \
-      hipLaunchKernelGGL(K64, g_, dim3(256), 0, s, __VA_ARGS__); \
+    const dim3 g_(rows_grid_rl((w), rl_, (BLK)));                   \
+    if (rl_ == 16)                                                  \
+      hipLaunchKernelGGL(K16, g_, dim3(BLK), 0, s, __VA_ARGS__);    \
+    else if (rl_ == 32)                                             \
+      hipLaunchKernelGGL(K32, g_, dim3(BLK), 0, s, __VA_ARGS__);    \
+    else                                                            \
+      hipLaunchKernelGGL(K64, g_, dim3(BLK), 0, s, __VA_ARGS__);    \
   } while (0)
+#define D4_LAUNCH_ROWS(K16, K32, K64, s, p, w, ...) \
+  D4_LAUNCH_ROWS_B(256, K16, K32, K64, s, p, w, __VA_ARGS__)
 
 inline unsigned rows_grid(const Work& w) {
   return (unsigned)((w.B * w.N + kWaves - 1) / kWaves);
@@ -2427,15 +2429,21 @@ hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
 template <bool S, int PART>
 void launch_pairs_model(hipStream_t s, const DevTables& t, const Params& p, const Inputs& in,
                         const Work& w, bool grad, double* grad_out) {
+  // the two-body pass runs beside the three-body kernel: one-wave
+  // work-groups take a SIMD as soon as one of its three-body waves ends
+  // (a four-wave group needs four such SIMDs of one CU at once)
+  constexpr int blk = PART == kPartE2 ? 64 : 256;
   if (grad && p.pgrad)
-    D4_LAUNCH_ROWS((k_pairs<true, true, S, 16, PART>), (k_pairs<true, true, S, 32, PART>),
-                   (k_pairs<true, true, S, 64, PART>), s, p, w, t, p, in, w, grad_out);
+    D4_LAUNCH_ROWS_B(blk, (k_pairs<true, true, S, 16, PART>), (k_pairs<true, true, S, 32, PART>),
+                     (k_pairs<true, true, S, 64, PART>), s, p, w, t, p, in, w, grad_out);
   else if (grad)
-    D4_LAUNCH_ROWS((k_pairs<true, false, S, 16, PART>), (k_pairs<true, false, S, 32, PART>),
-                   (k_pairs<true, false, S, 64, PART>), s, p, w, t, p, in, w, grad_out);
+    D4_LAUNCH_ROWS_B(blk, (k_pairs<true, false, S, 16, PART>),
+                     (k_pairs<true, false, S, 32, PART>), (k_pairs<true, false, S, 64, PART>), s,
+                     p, w, t, p, in, w, grad_out);
   else
-    D4_LAUNCH_ROWS((k_pairs<false, false, S, 16, PART>), (k_pairs<false, false, S, 32, PART>),
-                   (k_pairs<false, false, S, 64, PART>), s, p, w, t, p, in, w, grad_out);
+    D4_LAUNCH_ROWS_B(blk, (k_pairs<false, false, S, 16, PART>),
+                     (k_pairs<false, false, S, 32, PART>), (k_pairs<false, false, S, 64, PART>),
+                     s, p, w, t, p, in, w, grad_out);
 }
 
 template <int PART>
