@@ -2258,6 +2258,112 @@ __global__ __launch_bounds__(256) void k_chain(DevTables t, Params p, Inputs in,
 }
 
 // --------------------------------------------------------------------------
+// chain pass for molecules of up to kAtm1MaxN atoms: the same pair terms as
+// k_chain, each unordered pair once. The coefficient of a pair is symmetric
+// (dL/dCN_i + dL/dCN_j, beta_i + beta_j, lam_i q_j + lam_j q_i and the
+// element-pair constants), so x gets +c (r_x - r_y) and y gets -c (r_x - r_y).
+// One work-group per molecule; each of its kChainWaves waves takes every
+// kChainWaves-th 8 x 8 tile (bx <= by) on its own (no work-group barrier in
+// the loop): row sums by DPP within the 8-lane groups, column sums by lane
+// shuffles across them, both in a fixed butterfly order, added into the
+// wave's own per-atom accumulators in LDS; the waves' accumulators are added
+// in wave order at the end (no atomics). Atom data are staged in LDS once.
+// --------------------------------------------------------------------------
+
+constexpr int kChainAtom = 8;   // doubles per staged atom: x, y, z, dL/dCN, beta, lam, q, Z
+constexpr int kChainWaves = 8;  // waves per work-group
+
+__global__ __launch_bounds__(64 * kChainWaves) void k_chain_sym(DevTables t, Params p,
+                                                                  Inputs in, Work w,
+                                                                  double* grad_out) {
+  extern __shared__ double s_dyn[];  // [n][kChainAtom] atoms, [kChainWaves][3][n] sums
+  const int64_t m = blockIdx.x;
+  const int64_t N = w.N;
+  const int n = w.natom[m];
+  if (n < 2) return;
+  constexpr int NT = 64 * kChainWaves;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tx = lane & 7, ty = lane >> 3;
+  const int64_t* Z = in.numbers + m * N;
+  const double* R = in.positions + m * N * 3;
+  const bool eeq = in.q_in == nullptr;
+  double* s_at = s_dyn;
+  double* s_f = s_dyn + (size_t)kChainAtom * n;
+  for (int i = tid; i < n; i += NT) {
+    const int64_t row = m * N + i;
+    double* at = s_at + kChainAtom * i;
+    at[0] = R[3 * i];
+    at[1] = R[3 * i + 1];
+    at[2] = R[3 * i + 2];
+    at[3] = w.dldcn[row];
+    at[4] = eeq ? w.beta[row] : 0.0;
+    at[5] = eeq ? w.lam[row] : 0.0;
+    at[6] = eeq ? w.q[row] : 0.0;
+    at[7] = (double)Z[i];
+  }
+  for (int e = tid; e < kChainWaves * 3 * n; e += NT) s_f[e] = 0.0;
+  __syncthreads();
+  const int nb = (n + 7) >> 3;
+  const int tiles = nb * (nb + 1) / 2;
+  double* f = s_f + (size_t)wv * 3 * n;
+  for (int tl = wv; tl < tiles; tl += kChainWaves) {
+    int by = (int)((sqrt(8.0 * tl + 1.0) - 1.0) * 0.5);
+    while (by * (by + 1) / 2 > tl) --by;
+    while ((by + 1) * (by + 2) / 2 <= tl) ++by;
+    const int bx = tl - by * (by + 1) / 2;
+    const int x = 8 * bx + tx, y = 8 * by + ty;
+    double fc[3] = {0.0, 0.0, 0.0};
+    if (x < y && y < n) {
+      const double* ax = s_at + kChainAtom * x;
+      const double* ay = s_at + kChainAtom * y;
+      const int zx = (int)ax[7], zy = (int)ay[7];
+      if (zx != 0 && zy != 0) {
+        const double dx = ax[0] - ay[0], dy = ax[1] - ay[1], dz = ax[2] - ay[2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ir = rsqrt_nt(r2);
+        const double r = r2 * ir;
+        const PairConst pc = pair_const(t, zx, zy);
+        const double dc = erf_count_dr(r, pc.irc);
+        double coef = 0.0;
+        if (r <= p.cutcn) coef += (ax[3] + ay[3]) * pc.enf * dc;
+        if (eeq) {
+          if (r <= p.cuteeq) coef += (ax[4] + ay[4]) * dc;
+          // d/dr erf(gamma r) / r
+          const double gm = pc.gam;
+          const double dA =
+              ir * fma(2.0 * kInvSqrtPi * gm, exp(-gm * gm * r2), -erf(gm * r) * ir);
+          coef -= (ax[5] * ay[6] + ay[5] * ax[6]) * dA;
+        }
+        coef *= ir;
+        fc[0] = coef * dx;
+        fc[1] = coef * dy;
+        fc[2] = coef * dz;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      // y atom 8 by + ty: minus the sum over its row's eight lanes
+      const double rs = sum_lanes8(fc[c]);
+      // x atom 8 bx + tx: the sum over the eight rows
+      double cs = fc[c];
+      cs += __shfl_xor(cs, 8);
+      cs += __shfl_xor(cs, 16);
+      cs += __shfl_xor(cs, 32);
+      if (tx == 0 && y < n) f[c * n + y] -= rs;
+      if (ty == 0 && x < n) f[c * n + x] += cs;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < 3 * n; e += NT) {
+    const int c = e / n, i = e - c * n;
+    if (s_at[kChainAtom * i + 7] == 0.0) continue;
+    double g = 0.0;
+    for (int k = 0; k < kChainWaves; ++k) g += s_f[(size_t)k * 3 * n + e];
+    grad_out[(m * N + i) * 3 + c] += g;
+  }
+}
+
+// --------------------------------------------------------------------------
 // properties: C6 matrix with charge-dependent weights and static
 // polarisabilities (disp.py:189-197), one wave per atom row
 // --------------------------------------------------------------------------
@@ -2597,6 +2703,15 @@ hipError_t launch_atm_rows(hipStream_t s, const Params& p, const Inputs& in, con
 
 hipError_t launch_chain(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, double* grad_out) {
+  if (w.N <= kAtm1MaxN) {
+    // at most (8 + 3 * 8) * 256 doubles = 64 KiB, the default dynamic LDS limit
+    static_assert((kChainAtom + 3 * kChainWaves) * kAtm1MaxN * sizeof(double) <= 65536, "LDS");
+    if (w.B == 0) return hipSuccess;
+    const size_t shmem = sizeof(double) * (size_t)(kChainAtom + 3 * kChainWaves) * (size_t)w.N;
+    hipLaunchKernelGGL(k_chain_sym, dim3((unsigned)w.B), dim3(64 * kChainWaves), shmem, s, t, p,
+                       in, w, grad_out);
+    return hipGetLastError();
+  }
   D4_LAUNCH_ROWS((k_chain<16>), (k_chain<32>), (k_chain<64>), s, p, w, t, p, in, w, grad_out);
   return hipGetLastError();
 }
