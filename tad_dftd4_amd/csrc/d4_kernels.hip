@@ -133,6 +133,93 @@ __device__ __forceinline__ double sum_lanes8(double v) {
   return v;
 }
 
+// Coordination numbers of molecules of up to 256 atoms, each unordered pair
+// once: the counting function and the electronegativity factor are
+// symmetric, so both atoms of a pair get the same terms. One work-group of
+// kCnWaves independent waves per molecule over the 8 x 8 tiles (bx <= by),
+// row and column sums in a fixed butterfly order into per-wave accumulators
+// in LDS, added in wave order at the end.
+constexpr int kCnWaves = 8;
+
+__global__ __launch_bounds__(64 * kCnWaves) void k_cn_sym(DevTables t, Params p, Inputs in,
+                                                            Work w) {
+  extern __shared__ double s_dyn[];  // [n][4] x, y, z, Z; [kCnWaves][2][n] sums
+  constexpr int NT = 64 * kCnWaves;
+  const int64_t m = blockIdx.x;
+  const int64_t N = w.N;
+  const int n = w.natom[m];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tx = lane & 7, ty = lane >> 3;
+  const int64_t* Z = in.numbers + m * N;
+  const double* R = in.positions + m * N * 3;
+  double* s_at = s_dyn;
+  double* s_f = s_dyn + (size_t)4 * n;
+  for (int i = tid; i < n; i += NT) {
+    double* at = s_at + 4 * i;
+    at[0] = R[3 * i];
+    at[1] = R[3 * i + 1];
+    at[2] = R[3 * i + 2];
+    at[3] = (double)Z[i];
+  }
+  for (int e = tid; e < kCnWaves * 2 * n; e += NT) s_f[e] = 0.0;
+  __syncthreads();
+  const int nb = (n + 7) >> 3;
+  const int tiles = nb * (nb + 1) / 2;
+  double* f = s_f + (size_t)wv * 2 * n;
+  for (int tl = wv; tl < tiles; tl += kCnWaves) {
+    int by = (int)((sqrt(8.0 * tl + 1.0) - 1.0) * 0.5);
+    while (by * (by + 1) / 2 > tl) --by;
+    while ((by + 1) * (by + 2) / 2 <= tl) ++by;
+    const int bx = tl - by * (by + 1) / 2;
+    const int x = 8 * bx + tx, y = 8 * by + ty;
+    double v[2] = {0.0, 0.0};  // D4 CN term, EEQ CN term
+    if (x < y && y < n) {
+      const double* ax = s_at + 4 * x;
+      const double* ay = s_at + 4 * y;
+      const int zx = (int)ax[3], zy = (int)ay[3];
+      if (zx != 0 && zy != 0) {
+        const double dx = ax[0] - ay[0], dy = ax[1] - ay[1], dz = ax[2] - ay[2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r = r2 * rsqrt_nt(r2);
+        const PairConst pc = pair_const(t, zx, zy);
+        const double c = erf_count(r, pc.irc);
+        if (r <= p.cutcn) v[0] = pc.enf * c;
+        if (r <= p.cuteeq) v[1] = c;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const double rs = sum_lanes8(v[c]);  // y atom 8 by + ty
+      double cs = v[c];                    // x atom 8 bx + tx
+      cs += __shfl_xor(cs, 8);
+      cs += __shfl_xor(cs, 16);
+      cs += __shfl_xor(cs, 32);
+      if (tx == 0 && y < n) f[c * n + y] += rs;
+      if (ty == 0 && x < n) f[c * n + x] += cs;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < N; i += NT) {
+    const int64_t row = m * N + i;
+    if (i >= n || s_at[4 * i + 3] == 0.0) {
+      w.cn[row] = 0.0;
+      w.cnq[row] = 0.0;
+      w.dcnq[row] = 0.0;
+      continue;
+    }
+    double s4 = 0.0, se = 0.0;
+    for (int k = 0; k < kCnWaves; ++k) {
+      s4 += s_f[(size_t)k * 2 * n + i];
+      se += s_f[(size_t)k * 2 * n + n + i];
+    }
+    w.cn[row] = s4;
+    // smooth cap: log(1 + e^cmax) - log(1 + e^(cmax - cn))
+    const double ex = exp(kCnMax - se);
+    w.cnq[row] = log(1.0 + exp(kCnMax)) - log(1.0 + ex);
+    w.dcnq[row] = ex / (1.0 + ex);
+  }
+}
+
 // part kPairsAtm writes the q = 0 weights (gw0, dgw0), kPairsTwoBody those at
 // the EEQ charges (gw, dgw, dgwq), kPairsBoth all of them.
 __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
@@ -2509,6 +2596,13 @@ hipError_t launch_prep(hipStream_t s, const Inputs& in, const Work& w) {
 
 hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
                      const Inputs& in, const Work& w) {
+  if (w.N <= kAtm1MaxN) {
+    if (w.B == 0) return hipSuccess;
+    const size_t shmem = sizeof(double) * (size_t)(4 + 2 * kCnWaves) * (size_t)w.N;
+    hipLaunchKernelGGL(k_cn_sym, dim3((unsigned)w.B), dim3(64 * kCnWaves), shmem, s, t, p, in,
+                       w);
+    return hipGetLastError();
+  }
   D4_LAUNCH_ROWS((k_cn<16>), (k_cn<32>), (k_cn<64>), s, p, w, t, p, in, w);
   return hipGetLastError();
 }
