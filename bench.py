@@ -374,7 +374,18 @@ def main() -> int:
     runtime.profile_enable(dev, True, slots=("atm",))
     timed(step, args.steps)
     runtime.profile_enable(dev, False)
+    prof_step = runtime.profile_read(dev)
+    # the roofline figure: the same steps on one stream, so that the
+    # three-body kernel runs with no other kernel beside it (in the two-stream
+    # step the other side's kernels share its CUs by design, which stretches
+    # its own duration)
+    one = args.one_stream
+    kp.one_stream = True
+    runtime.profile_enable(dev, True, slots=("atm",))
+    timed(step, args.steps)
+    runtime.profile_enable(dev, False)
     prof = runtime.profile_read(dev)
+    kp.one_stream = one
     # per-kernel averages from a fully instrumented pass of the same steps
     runtime.profile_enable(dev, True)
     for _ in range(args.steps):
@@ -384,8 +395,15 @@ def main() -> int:
     # the headline: the step captured once in a HIP graph (both streams'
     # launch sequences) and replayed; K replays timed after W untimed ones
     cap = runtime.CapturedStep(numbers, positions, charge, kp, forces=forces)
-    for _ in range(args.warmup):
+    # W untimed replays, and more until 0.2 s of replays have run: the first
+    # few dozen replays after the capture run ~5 % slower (clocks settling)
+    t_warm = time.perf_counter()
+    warm_replays = 0
+    while warm_replays < args.warmup or time.perf_counter() - t_warm < 0.2:
         cap.replay()
+        warm_replays += 1
+        if warm_replays % 10 == 0:
+            torch.cuda.synchronize()
     ms_total = timed(cap.replay, args.steps)
     ms_step = ms_total / args.steps
     value = B * args.steps / (ms_total / 1e3)
@@ -395,6 +413,7 @@ def main() -> int:
     work = metrics.workload_flops(mols, CUT[0], CUT[1], with_forces=forces)
     atm_ms, atm_n = prof["atm"]
     atm_avg_s = atm_ms / max(atm_n, 1) / 1e3
+    atm_step_ms = prof_step["atm"][0] / max(prof_step["atm"][1], 1)
     achieved = work["atm"] / atm_avg_s / 1e12
     kernel_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in prof_all.items() if v[1]}
 
@@ -442,6 +461,11 @@ def main() -> int:
             "traffic": traffic_from_profiles(args.workload),
             "algorithmic_flops_per_launch": work["atm"],
             "avg_launch_ms": round(atm_avg_s * 1e3, 4),
+            "timing": "HIP events around each launch of the kernel on its stream, in eager "
+                      "steps on one stream (the kernel alone on the GPU)",
+            "in_step_avg_launch_ms": round(atm_step_ms, 4),
+            "in_step_note": "the same kernel in the two-stream step, sharing the CUs with the "
+                            "two-body side",
         },
         "step_flops": {
             "algorithmic_tflops": round(work["total"] / (ms_step / 1e3) / 1e12, 3),
@@ -452,6 +476,7 @@ def main() -> int:
                   "value": round(B * args.steps / (ms_eager / 1e3), 1),
                   "note": "the same step launched kernel by kernel from the host"},
         "timed_path": "HIP graph replay of the captured step (both streams)",
+        "graph_warmup_replays": warm_replays,
         "graph_equals_eager": same,
         "kernel_avg_ms": kernel_ms,
         "kernel_avg_note": "per-kernel HIP-event averages from an eager pass of the same steps "

@@ -141,15 +141,18 @@ constexpr int kAtm1Acc = 5;  // energy share, gradient (3), dL/dCN
 __host__ __device__ inline int atm1_splits(int n);
 // The 8 x 8 variant (one wave per work-group, kAtm8-atom blocks; the
 // gradient passes) splits a molecule over atm8_splits(n) waves, also a
-// function of n alone: about one 32-atom molecule of work each, at most 64
-// (each split writes a partial sum per atom).
+// function of n alone: about one 40-atom molecule of work (n^3 = 65536) each,
+// at most 64 (each split writes a partial sum per atom). On a 1024 x 20-120
+// batch, 32768 per split gave the same kernel time alone but a 2.5 % slower
+// two-stream step (twice the waves and partial sums); 49152 and 81920 or
+// more were slower.
 constexpr int kAtm8 = 8;
 constexpr int kAtm8MaxSplits = 64;
 __host__ __device__ inline int atm8_splits(int n) {
   const int nb = (n + kAtm8 - 1) / kAtm8;
   const int tiles = nb * (nb + 1) / 2;
   const long long work = (long long)n * n * n;
-  int s = (int)((work + 32767) / 32768);
+  int s = (int)((work + 65535) / 65536);
   s = s > kAtm8MaxSplits ? kAtm8MaxSplits : s;
   return tiles < 1 ? 1 : (s < 1 ? 1 : (s > tiles ? tiles : s));
 }

@@ -1724,7 +1724,12 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
       }
     }
     if (GRAD) {
-      const double W6 = FULL ? 2.0 * (q.gxy + rz.w) : fma(q.gx, sx, fma(q.gy, sy, rz.w * sz));
+      // the fused energy + gradient call has dL/dE = 1 for every atom (its
+      // triples without atoms have e = 0, whatever the weight)
+      constexpr bool kUnit = ENERGY && GRAD && !PG;
+      const double W6 = kUnit ? (FULL ? 6.0 : sx + sy + sz)
+                              : (FULL ? 2.0 * (q.gxy + rz.w)
+                                      : fma(q.gx, sx, fma(q.gy, sy, rz.w * sz)));
       const double We = W6 * e;
       const double H = fma(g, fma(alp * T, f, -1.5), -G2s);
       // ds/da = 2 a W3 - X Y, ds/db = X Y + 2 (c - b) W3, ds/dc = X Y - 2 (c - b) W3
@@ -2676,6 +2681,7 @@ hipError_t launch_atm8(hipStream_t s, const Params& p, const Inputs& in, const W
   if (grid == 0) return hipSuccess;
   const size_t shmem = sizeof(double) * kAtm1Acc * (size_t)w.N;
   const dim3 g((unsigned)grid), blk(64);
+  if (energy && grad && in.gE != nullptr) return hipErrorInvalidValue;  // unit dL/dE only
   if (energy && grad)
     hipLaunchKernelGGL((k_atm8<true, true, false>), g, blk, shmem, s, p, in, w, S, nblocks);
   else if (energy)
