@@ -22,7 +22,7 @@ struct d4_event_pair {
 // fork and join it.
 struct d4_aux {
   hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr, eeq = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // Scratch of d4_energy_gradient_noalloc: the workspace of that entry point,
@@ -231,7 +231,6 @@ hipError_t companion(d4_context* ctx, hipStream_t s, d4_aux* out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&a.fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&a.join, hipEventDisableTiming);
-  if (e == hipSuccess) e = hipEventCreateWithFlags(&a.eeq, hipEventDisableTiming);
   (void)hipSetDevice(prev);
   if (e != hipSuccess) return e;
   ctx->aux.emplace(s, a);
@@ -310,18 +309,14 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
   D4_RUN_AUX(D4_SLOT_PAIRS,
              d4::launch_pairs(c.aux.s, ctx->t, c.p, c.in, c.w, grad, nullptr, d4::kPairsAtm),
              "three-body pair factors");
+  D4_RUN_AUX(D4_SLOT_ATM, d4::launch_atm(c.aux.s, c.p, c.in, c.w, energy, grad), "three-body term");
+  // The three-body kernel's one-wave work-groups (two per SIMD) take every
+  // slot they free, so a four-wave work-group launched on the other stream
+  // behind them can wait for most of its run: the EEQ solve is one launch
+  // (its pivoted-LU fallback runs inside it) and the two-body pass uses
+  // one-wave work-groups.
   if (eeq) {
     D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
-    // the three-body kernel (one-wave work-groups, two per SIMD, that refill
-    // every slot they free) starts after the EEQ solve: a work-group of the
-    // EEQ's last kernel launched behind it waited ~0.5 ms for a free CU
-    if (!c.p.one_stream) {
-      D4_TRY(hipEventRecord(c.aux.eeq, c.s), "record EEQ event");
-      D4_TRY(hipStreamWaitEvent(c.aux.s, c.aux.eeq, 0), "wait for EEQ");
-    }
-  }
-  D4_RUN_AUX(D4_SLOT_ATM, d4::launch_atm(c.aux.s, c.p, c.in, c.w, energy, grad), "three-body term");
-  if (eeq) {
     D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsTwoBody),
            "reference weights");
     D4_RUN(D4_SLOT_WEIGHTS, d4::launch_pairvec(c.s, ctx->t, c.w, c.in, d4::kPairsTwoBody),
@@ -485,7 +480,6 @@ int d4_context_destroy(d4_context* ctx) {
   for (auto& kv : ctx->aux) {
     (void)hipStreamSynchronize(kv.second.s);
     (void)hipEventDestroy(kv.second.fork);
-    (void)hipEventDestroy(kv.second.eeq);
     (void)hipEventDestroy(kv.second.join);
     (void)hipStreamDestroy(kv.second.s);
   }

@@ -330,6 +330,9 @@ __device__ __forceinline__ int strict_row(int e) {
   return i;
 }
 
+__device__ void eeq_lu_molecule(const DevTables& t, const Inputs& in, const Work& w, int64_t m,
+                                double* s_val, int* s_idx);
+
 // Two work-groups per CU fit the LDS of a 120-atom factor; keep the register
 // use low enough for two waves per SIMD as well.
 template <int MODE>
@@ -428,7 +431,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   __syncthreads();
 
   if (p.force_lu || chol_factor<true>(L, n, rb, re, yblk, flag)) {
+    // a pivot <= 0 (or forced): the pivoted LU of the bordered system in the
+    // workspace, by this work-group (no separate launch behind this one)
     if (tid == 0) w.status[m] |= kStatusLU;
+    eeq_lu_molecule(t, in, w, m, red, reinterpret_cast<int*>(red + kWaves));
     return;
   }
   // e^T C^-1 b = y_e . y_b and e^T C^-1 e = y_e . y_e, so one backward
@@ -523,7 +529,38 @@ __device__ void lu_solve(const double* A, int ld, const int32_t* piv, int n1,
   __syncthreads();
 }
 
-// Pivoted LU of the bordered system of molecule m (work-group wide).
+// Largest |value| of the work-group and its index, the lowest index among
+// equal values (the same pivot whatever the reduction order); s_val, s_idx:
+// kWaves entries of LDS.
+__device__ __forceinline__ void block_argmax(double& v, int& i, double* s_val, int* s_idx) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(v, off, 64);
+    const int oi = __shfl_xor(i, off, 64);
+    if (ov > v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_val[wave] = v;
+    s_idx[wave] = i;
+  }
+  __syncthreads();
+  v = s_val[0];
+  i = s_idx[0];
+  for (int k = 1; k < kWaves; ++k) {
+    if (s_val[k] > v || (s_val[k] == v && s_idx[k] < i)) {
+      v = s_val[k];
+      i = s_idx[k];
+    }
+  }
+  __syncthreads();
+}
+
+// Pivoted LU of the bordered system of molecule m (work-group wide, 256
+// threads); s_val, s_idx: kWaves entries of LDS.
 __device__ void eeq_lu_molecule(const DevTables& t, const Inputs& in, const Work& w, int64_t m,
                                 double* s_val, int* s_idx) {
   const int n = w.natom[m];
@@ -547,23 +584,9 @@ __device__ void eeq_lu_molecule(const DevTables& t, const Inputs& in, const Work
         bi = i;
       }
     }
-    s_val[tid] = best;
-    s_idx[tid] = bi;
-    __syncthreads();
-    for (int off = 128; off >= 1; off >>= 1) {
-      if (tid < off) {
-        const double o = s_val[tid + off];
-        const int oi = s_idx[tid + off];
-        if (o > s_val[tid] || (o == s_val[tid] && oi < s_idx[tid])) {
-          s_val[tid] = o;
-          s_idx[tid] = oi;
-        }
-      }
-      __syncthreads();
-    }
-    const int pk = s_idx[0];
-    const double pv = s_val[0];
-    __syncthreads();
+    block_argmax(best, bi, s_val, s_idx);
+    const int pk = bi;
+    const double pv = best;
     if (!(pv > 0.0)) {
       singular = 1;
       break;
@@ -600,8 +623,8 @@ __device__ void eeq_lu_molecule(const DevTables& t, const Inputs& in, const Work
 // The molecules whose Cholesky pivot was <= 0 (or all, when forced): a few
 // work-groups walk the batch, so the launch costs little when none needs it.
 __global__ __launch_bounds__(256) void k_eeq_lu(DevTables t, Params p, Inputs in, Work w) {
-  __shared__ double s_val[256];
-  __shared__ int s_idx[256];
+  __shared__ double s_val[kWaves];
+  __shared__ int s_idx[kWaves];
   if (in.q_in != nullptr) return;
   for (int64_t m = blockIdx.x; m < w.B; m += gridDim.x) {
     if (!(w.status[m] & kStatusLU)) continue;  // uniform over the work-group
@@ -1148,7 +1171,8 @@ hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
   // left in the workspace)
   hipLaunchKernelGGL(k_eeq_order, dim3(1), dim3(1024), 0, s, w);
   hipError_t e;
-  if (eeq_tiled(p, w.N)) {
+  const bool tiled = eeq_tiled(p, w.N);
+  if (tiled) {
     e = launch_eeq_tiled(s, t, p, in, w);
   } else {
     const int mode = eeq_mode(w.N, kEeqVectors);
@@ -1159,8 +1183,9 @@ hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
       e = launch_lds(k_eeq_chol<kEeqFactorLds>, configured[0], grid, shmem, s, t, p, in, w);
   }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_eeq_lu, dim3(grid < kLuGroups ? grid : kLuGroups), dim3(256), 0, s, t, p,
-                     in, w);
+  if (tiled)  // (the LDS kernels solve their failed systems themselves)
+    hipLaunchKernelGGL(k_eeq_lu, dim3(grid < kLuGroups ? grid : kLuGroups), dim3(256), 0, s, t,
+                       p, in, w);
   return hipGetLastError();
 }
 
