@@ -127,6 +127,7 @@ struct Work {
   double* pg2;      // [B*N*5] two-body parameter-gradient partials per atom row
   double* pg3;      // [B*P*4] three-body parameter-gradient partials per work-group
   int32_t* eqord;   // [B] molecule order of the per-molecule EEQ kernels (largest first)
+  double* bbox2;    // [B] squared diagonal of a molecule's bounding box (N <= kAtm1MaxN)
   int64_t B, N;
 };
 
@@ -315,6 +316,7 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
   v.pg2 = (double*)take(8 * bn * 5);
   v.pg3 = (double*)take(32 * size_t(B) * size_t(pgrad_partials(N)));
   v.eqord = (int32_t*)take(4 * size_t(B));
+  v.bbox2 = (double*)take(8 * size_t(B));
   if (w) *w = v;
   return off;
 }

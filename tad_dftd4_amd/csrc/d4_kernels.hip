@@ -162,7 +162,35 @@ __global__ __launch_bounds__(64 * kCnWaves) void k_cn_sym(DevTables t, Params p,
     at[3] = (double)Z[i];
   }
   for (int e = tid; e < kCnWaves * 2 * n; e += NT) s_f[e] = 0.0;
-  __syncthreads();
+  {
+    // the molecule's bounding box (the three-body kernel reads its squared
+    // diagonal: a molecule inside the cutoff sphere has no screening)
+    __shared__ double s_bb[kCnWaves][6];
+    double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
+    for (int i = tid; i < n; i += NT) {
+      if (Z[i] == 0) continue;
+      lo0 = fmin(lo0, R[3 * i]);
+      hi0 = fmax(hi0, R[3 * i]);
+      lo1 = fmin(lo1, R[3 * i + 1]);
+      hi1 = fmax(hi1, R[3 * i + 1]);
+      lo2 = fmin(lo2, R[3 * i + 2]);
+      hi2 = fmax(hi2, R[3 * i + 2]);
+    }
+    const double bb[6] = {wave_max(hi0), wave_max(-lo0), wave_max(hi1),
+                          wave_max(-lo1), wave_max(hi2), wave_max(-lo2)};
+    if (lane == 0)
+      for (int k = 0; k < 6; ++k) s_bb[wv][k] = bb[k];
+    __syncthreads();
+    if (tid == 0) {
+      double b[6];
+      for (int k = 0; k < 6; ++k) {
+        b[k] = s_bb[0][k];
+        for (int q = 1; q < kCnWaves; ++q) b[k] = fmax(b[k], s_bb[q][k]);
+      }
+      const double d0 = b[0] + b[1], d1 = b[2] + b[3], d2 = b[4] + b[5];
+      w.bbox2[m] = d0 * d0 + d1 * d1 + d2 * d2;
+    }
+  }
   const int nb = (n + 7) >> 3;
   const int tiles = nb * (nb + 1) / 2;
   double* f = s_f + (size_t)wv * 2 * n;
@@ -1836,7 +1864,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
   // consecutive blocks of its XCD: neighbouring molecules share an XCD's L2,
   // and a batch sorted by size still spreads its large molecules over all
   // XCDs (with whole eighths of the batch per XCD, a sorted 1024 x 20-120
-  // batch took 1.86 ms per step instead of 1.10; now 1.06-1.10, random 1.11)
+  // batch took 1.86 ms per step instead of 1.10)
   const int64_t b = blockIdx.x, j = b >> 3, l = j / S;
   const int64_t m = (((l >> 3) << 3) + (b & 7)) * 8 + (l & 7);
   if (m >= nblocks) return;  // nblocks: the number of molecules
@@ -1850,28 +1878,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
   const int tiles = nw + (nb > 1 ? (nb - 1) * (nb - 2) / 2 : 0);
   const int lane = threadIdx.x, tx = lane & (T8 - 1), ty = lane >> 3;
   const int64_t* Z = in.numbers + m * N;
-  const double* R = in.positions + m * N * 3;
   const double c3sq = p.cut3sq;
   for (int e = lane; e < NA * n; e += 64) s_acc[e] = 0.0;
   Atm8Sums u = {};
   // molecules whose bounding box fits in the cutoff sphere have every pair
   // within the cutoff: all screening weights are 2 and their masks drop out
-  bool full;
-  {
-    double lo0 = 1e300, lo1 = 1e300, lo2 = 1e300, hi0 = -1e300, hi1 = -1e300, hi2 = -1e300;
-    for (int i = lane; i < n; i += 64) {
-      if (Z[i] == 0) continue;
-      lo0 = fmin(lo0, R[3 * i]);
-      hi0 = fmax(hi0, R[3 * i]);
-      lo1 = fmin(lo1, R[3 * i + 1]);
-      hi1 = fmax(hi1, R[3 * i + 1]);
-      lo2 = fmin(lo2, R[3 * i + 2]);
-      hi2 = fmax(hi2, R[3 * i + 2]);
-    }
-    const double d0 = wave_max(hi0) + wave_max(-lo0), d1 = wave_max(hi1) + wave_max(-lo1),
-                 d2 = wave_max(hi2) + wave_max(-lo2);
-    full = d0 * d0 + d1 * d1 + d2 * d2 <= c3sq;
-  }
+  // (the squared box diagonal comes from the coordination-number pass)
+  const bool full = w.bbox2[m] <= c3sq;
 
   // W-tile lane roles: lane (h, p) = (lane >> 5, lane & 31) owns the pair
   // p = v (v - 1) / 2 + u (u < v) of block 2 t + h; lanes p >= 28 are idle
