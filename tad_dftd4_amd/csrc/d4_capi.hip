@@ -256,8 +256,10 @@ hipError_t join_aux(Call& c) {
 // Sorted order, extents and coordination numbers on the call's stream.
 int run_cn(d4_context* ctx, Call& c) {
   D4_RUN(D4_SLOT_PREP, enter_sorted(c, false), "spatial order");
-  D4_RUN(D4_SLOT_PREP, d4::launch_prep(c.s, c.in, c.w), "prep");
-  if (!c.p.d4s) D4_RUN(D4_SLOT_PREP, d4::launch_zslots(c.s, c.in, c.w), "element slots");
+  if (c.w.N > d4::kAtm1MaxN) {  // (smaller batches: one launch does all three)
+    D4_RUN(D4_SLOT_PREP, d4::launch_prep(c.s, c.in, c.w), "prep");
+    if (!c.p.d4s) D4_RUN(D4_SLOT_PREP, d4::launch_zslots(c.s, c.in, c.w), "element slots");
+  }
   D4_RUN(D4_SLOT_CN, d4::launch_cn(c.s, ctx->t, c.p, c.in, c.w), "coordination numbers");
   return D4_OK;
 }

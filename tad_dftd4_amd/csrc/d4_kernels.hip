@@ -138,17 +138,76 @@ __device__ __forceinline__ double sum_lanes8(double v) {
 // symmetric, so both atoms of a pair get the same terms. One work-group of
 // kCnWaves independent waves per molecule over the 8 x 8 tiles (bx <= by),
 // row and column sums in a fixed butterfly order into per-wave accumulators
-// in LDS, added in wave order at the end.
+// in LDS, added in wave order at the end. The work-group first does what
+// k_prep and k_zslots do for its molecule (the extent n from the last
+// non-zero Z, the Z range check and, with ZS, the element slots), so the
+// step starts with one launch instead of three.
 constexpr int kCnWaves = 8;
 
+template <bool ZS>
 __global__ __launch_bounds__(64 * kCnWaves) void k_cn_sym(DevTables t, Params p, Inputs in,
                                                             Work w) {
   extern __shared__ double s_dyn[];  // [n][4] x, y, z, Z; [kCnWaves][2][n] sums
+  __shared__ int s_red[2][kCnWaves];
+  __shared__ int s_present[kMaxZ];
   constexpr int NT = 64 * kCnWaves;
   const int64_t m = blockIdx.x;
   const int64_t N = w.N;
-  const int n = w.natom[m];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int n;
+  {
+    const int64_t* z = in.numbers + m * N;
+    int last = -1, bad = 0;
+    for (int64_t i = tid; i < N; i += NT) {
+      const int64_t zi = z[i];
+      if (zi != 0) last = (int)i;
+      if (zi < 0 || zi >= kMaxZ) bad = 1;
+    }
+    if (ZS)
+      for (int e = tid; e < kMaxZ; e += NT) s_present[e] = 0;
+    last = wave_max(last);
+    bad = wave_or(bad);
+    if (lane == 0) {
+      s_red[0][wv] = last;
+      s_red[1][wv] = bad;
+    }
+    __syncthreads();
+    for (int k = 1; k < kCnWaves; ++k) {
+      last = max(last, s_red[0][k]);
+      bad |= s_red[1][k];
+    }
+    last = max(last, s_red[0][0]);
+    bad |= s_red[1][0];
+    n = bad ? 0 : last + 1;
+    if (tid == 0) {
+      w.natom[m] = n;
+      w.status[m] = bad ? kStatusBadZ : 0;
+    }
+    if (ZS) {
+      // element slots: the molecule's distinct elements in increasing Z
+      for (int i = tid; i < n; i += NT) {
+        const int64_t zi = z[i];
+        if (zi > 0) s_present[zi] = 1;
+      }
+      __syncthreads();
+      if (wv == 0) {
+        const unsigned long long lo = __ballot(s_present[lane] != 0);
+        const unsigned long long hi = __ballot(lane + 64 < kMaxZ && s_present[lane + 64] != 0);
+        const int count = __popcll(lo) + __popcll(hi);
+        const bool ok = count <= kZSlots;
+        for (int zz = lane; zz < kMaxZ; zz += 64) {
+          const unsigned long long below =
+              zz < 64 ? (lo & ((1ull << zz) - 1ull))
+                      : (hi & ((1ull << (zz - 64)) - 1ull));
+          const int sl = (zz < 64 ? 0 : __popcll(lo)) + __popcll(below);
+          const bool here = ok && zz > 0 && s_present[zz] != 0;
+          if (here) w.zlist[m * kZSlots + sl] = zz;
+          w.zslot[m * kMaxZ + zz] = here ? sl : -1;
+        }
+        if (lane == 0) w.nzs[m] = ok ? count : -1;
+      }
+    }
+  }
   const int tx = lane & 7, ty = lane >> 3;
   const int64_t* Z = in.numbers + m * N;
   const double* R = in.positions + m * N * 3;
@@ -2617,8 +2676,13 @@ hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
   if (w.N <= kAtm1MaxN) {
     if (w.B == 0) return hipSuccess;
     const size_t shmem = sizeof(double) * (size_t)(4 + 2 * kCnWaves) * (size_t)w.N;
-    hipLaunchKernelGGL(k_cn_sym, dim3((unsigned)w.B), dim3(64 * kCnWaves), shmem, s, t, p, in,
-                       w);
+    // (also the extents, the Z check and, for D4, the element slots)
+    if (p.d4s)
+      hipLaunchKernelGGL(k_cn_sym<false>, dim3((unsigned)w.B), dim3(64 * kCnWaves), shmem, s, t,
+                         p, in, w);
+    else
+      hipLaunchKernelGGL(k_cn_sym<true>, dim3((unsigned)w.B), dim3(64 * kCnWaves), shmem, s, t,
+                         p, in, w);
     return hipGetLastError();
   }
   D4_LAUNCH_ROWS((k_cn<16>), (k_cn<32>), (k_cn<64>), s, p, w, t, p, in, w);
