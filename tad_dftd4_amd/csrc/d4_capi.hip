@@ -305,8 +305,6 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
   if (eeq) {
     D4_RUN_AUX(D4_SLOT_WEIGHTS, d4::launch_weights(c.aux.s, ctx->t, c.p, c.w, c.in, d4::kPairsAtm),
                "reference weights (q = 0)");
-    D4_RUN_AUX(D4_SLOT_WEIGHTS, d4::launch_pairvec(c.aux.s, ctx->t, c.w, c.in, d4::kPairsAtm),
-               "C6 contraction vectors (q = 0)");
   }
   D4_RUN_AUX(D4_SLOT_PAIRS,
              d4::launch_pairs(c.aux.s, ctx->t, c.p, c.in, c.w, grad, nullptr, d4::kPairsAtm),
@@ -321,8 +319,6 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
     D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
     D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsTwoBody),
            "reference weights");
-    D4_RUN(D4_SLOT_WEIGHTS, d4::launch_pairvec(c.s, ctx->t, c.w, c.in, d4::kPairsTwoBody),
-           "C6 contraction vectors");
   }
   D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, grad, g_out, d4::kPairsTwoBody),
          "two-body term");

@@ -348,8 +348,6 @@ hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
 // Element slots of every molecule (after launch_prep), and the C6 contraction
 // vectors of one part (after its weights; D4 model only).
 hipError_t launch_zslots(hipStream_t s, const Inputs& in, const Work& w);
-hipError_t launch_pairvec(hipStream_t s, const DevTables& t, const Work& w, const Inputs& in,
-                          int part);
 hipError_t launch_pairs(hipStream_t s, const DevTables& t, const Params& p,
                         const Inputs& in, const Work& w, bool grad,
                         double* grad_out, int part);

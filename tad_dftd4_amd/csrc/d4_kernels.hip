@@ -309,6 +309,12 @@ __global__ __launch_bounds__(64 * kCnWaves) void k_cn_sym(DevTables t, Params p,
 
 // part kPairsAtm writes the q = 0 weights (gw0, dgw0), kPairsTwoBody those at
 // the EEQ charges (gw, dgw, dgwq), kPairsBoth all of them.
+// CV (D4, part q = 0 or part q alone): also the atom's C6 contraction
+// vectors, for each element slot e of its molecule v[a] = sum_b
+// RC6[Z_e][Z_j][a][b] gw_j[b] (a < 7; entry 7 is zero) from the weights just
+// formed (lane a walks the slots), which the pair pass reads in place of the
+// per-pair 7 x 7 reference blocks.
+template <bool CV>
 __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
                                                  Inputs in, int part) {
   const bool atm = part != kPairsTwoBody, two = part != kPairsAtm;
@@ -384,6 +390,28 @@ __global__ __launch_bounds__(256) void k_weights(DevTables t, Params p, Work w,
   if (atm) {
     w.gw0[o] = z0 * g;
     w.dgw0[o] = z0 * dg;
+  }
+  if (CV) {
+    // every lane of the atom's 8-lane group gathers the atom's weights of
+    // the part (uniform per group: the group's lanes are all active here)
+    const double gsel = atm ? z0 * g : zq * g;
+    const int base = (threadIdx.x & 63) & ~7;
+    double gg[kMaxRef];
+#pragma unroll
+    for (int b = 0; b < kMaxRef; ++b) gg[b] = __shfl(gsel, base + b, 64);
+    const int nz = w.nzs[m];
+    if (z == 0 || nz <= 0) return;
+    double* out = (atm ? w.cv0 : w.cv) + row * kZSlots * 8 + a;
+    for (int e = 0; e < nz; ++e) {
+      double v = 0.0;
+      if (a < kMaxRef) {
+        const double* blk =
+            t.rc6 + ((size_t)w.zlist[m * kZSlots + e] * kMaxZ + z) * kRc6Block + a * kMaxRef;
+#pragma unroll
+        for (int b = 0; b < kMaxRef; ++b) v += blk[b] * gg[b];
+      }
+      out[e * 8] = v;
+    }
   }
 }
 
@@ -488,37 +516,6 @@ __global__ __launch_bounds__(64) void k_zslots(Inputs in, Work w) {
     w.zslot[m * kMaxZ + z] = slot;
   }
   w.nzs[m] = ok ? count : -1;
-}
-
-// C6 contraction vectors: for atom j and each element slot e of its molecule,
-// v[a] = sum_b RC6[Z_e][Z_j][a][b] gw_j[b] (a < 7; entry 7 is zero), with the
-// weights at q (two-body part) or q = 0 (three-body part). Eight lanes per
-// atom, lane a walks the slots.
-__global__ __launch_bounds__(256) void k_pairvec(DevTables t, Work w, Inputs in, int part) {
-  const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t row = tid >> 3;
-  const int a = (int)(tid & 7);
-  if (row >= w.B * w.N) return;
-  const int64_t m = row / w.N;
-  const int j = (int)(row - m * w.N);
-  const int nz = w.nzs[m];
-  const int zj = j < w.natom[m] ? (int)in.numbers[row] : 0;
-  if (zj == 0 || nz <= 0) return;
-  const double* gwj = (part == kPairsAtm ? w.gw0 : w.gw) + row * kRefStride;
-  double g[kMaxRef];
-#pragma unroll
-  for (int b = 0; b < kMaxRef; ++b) g[b] = gwj[b];
-  double* out = (part == kPairsAtm ? w.cv0 : w.cv) + row * kZSlots * 8 + a;
-  for (int e = 0; e < nz; ++e) {
-    double v = 0.0;
-    if (a < kMaxRef) {
-      const double* blk = t.rc6 + ((size_t)w.zlist[m * kZSlots + e] * kMaxZ + zj) * kRc6Block +
-                          a * kMaxRef;
-#pragma unroll
-      for (int b = 0; b < kMaxRef; ++b) v += blk[b] * g[b];
-    }
-    out[e * 8] = v;
-  }
 }
 
 // Atom i of a pair (the atom whose sums are accumulated).
@@ -2694,17 +2691,13 @@ hipError_t launch_zslots(hipStream_t s, const Inputs& in, const Work& w) {
   return hipGetLastError();
 }
 
-hipError_t launch_pairvec(hipStream_t s, const DevTables& t, const Work& w, const Inputs& in,
-                          int part) {
-  const unsigned grid = (unsigned)((w.B * w.N * 8 + 255) / 256);
-  hipLaunchKernelGGL(k_pairvec, dim3(grid), dim3(256), 0, s, t, w, in, part);
-  return hipGetLastError();
-}
-
 hipError_t launch_weights(hipStream_t s, const DevTables& t, const Params& p,
                           const Work& w, const Inputs& in, int part) {
   const unsigned grid = (unsigned)((w.B * w.N * 8 + 255) / 256);
-  hipLaunchKernelGGL(k_weights, dim3(grid), dim3(256), 0, s, t, p, w, in, part);
+  if (part != kPairsBoth && !p.d4s)  // with the contraction vectors of the part
+    hipLaunchKernelGGL(k_weights<true>, dim3(grid), dim3(256), 0, s, t, p, w, in, part);
+  else
+    hipLaunchKernelGGL(k_weights<false>, dim3(grid), dim3(256), 0, s, t, p, w, in, part);
   return hipGetLastError();
 }
 
