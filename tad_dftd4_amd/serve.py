@@ -5,30 +5,45 @@ Requests (one molecule each: atomic numbers, positions in bohr, total charge,
 energy-only or energy + forces) arrive on the host at any time. The loop
 
 - keeps the element tables resident on the device (one context per device);
-- groups waiting requests by kind and padded size bucket (multiples of 32
-  atoms up to 192, the largest size the single-work-group EEQ solver takes,
-  then 256, then multiples of 128), so a batch holds many requests while
-  padding stays moderate (padding rows exit at once in every kernel);
+- groups waiting requests by padded size bucket (multiples of 32 atoms up to
+  192, the largest size the single-work-group EEQ solver takes, then 256,
+  then multiples of 128), so a batch holds many requests while padding stays
+  moderate (padding rows exit at once in every kernel); by default
+  (``merge_kinds``) energy-only requests share the energy + forces batches of
+  their bucket and their forces are dropped, which halves the number of
+  queues a request waits behind;
 - runs every batch in one of ``max_inflight`` slots, each with its own HIP
   stream and its own device arena and pinned host staging buffers, kept
   between batches (no allocation in steady state); a slot's share of the
   memory budget bounds the batch size (``d4_workspace_bytes`` + inputs +
   outputs);
-- packs a batch with vectorised copies into the pinned buffer and moves it
-  with one host-to-device copy, and the results with one copy back;
+- pads a batch of c requests to a count class (1, 2, 3, 4, 6, 8, 12, ...:
+  at most 1.5 c) with copies of a 4-atom filler molecule whose results are
+  dropped, so that every batch of one (count class, bucket, kind) in a slot
+  has the same layout, whose buffer views are made once; with ``graphs``
+  the first such batch runs eagerly and is then captured into a HIP graph
+  (both streams of the evaluation) that later ones replay. Graph replay is
+  off by default: on a Poisson stream new layouts keep appearing, each
+  capture synchronises the device, and the measured latency was several
+  times that of the eager launches (which suits a loop whose batch shapes
+  repeat);
+- packs a batch with one copy per request into the pinned buffer and moves
+  it with one host-to-device copy, and the results with one copy back;
 - reports each result when its batch's completion event has fired;
 - fails a bad request alone: a request with no atoms, atomic numbers
-  outside 0..103, positions of the wrong shape or not finite, or a footprint
-  above one slot's memory budget is answered at submission with a failed
-  ``Result`` (``error`` set); after a batch, a molecule whose status word
-  reports a singular EEQ system or whose energies are not finite gets a
-  failed result while the rest of its batch succeeds.
+  outside 0..103 or all zero, positions of the wrong shape or not finite, or
+  a footprint above one slot's memory budget is answered at submission (or
+  when its batch is packed, for the value checks) with a failed ``Result``
+  (``error`` set); after a batch, a molecule whose status word reports a
+  singular EEQ system or whose energies are not finite gets a failed result
+  while the rest of its batch succeeds.
 
 The kernels give every molecule a result that does not depend on its batch
-neighbours, nor on the padded length within one of the buckets up to 192
-atoms, so such a request's result is bitwise the same alone and in the mix;
-from 193 atoms on the padded batch takes the spatial order and the tiled EEQ
-and the results agree with the request alone to rounding.
+neighbours (nor on the batch size), nor on the padded length within one of
+the buckets up to 192 atoms, so such a request's result is bitwise the same
+alone and in the mix; from 193 atoms on the padded batch takes the spatial
+order and the tiled EEQ and the results agree with the request alone to
+rounding.
 """
 
 from __future__ import annotations
@@ -46,7 +61,7 @@ from . import runtime
 from .cutoff import Cutoff
 from .disp import kernel_params
 
-__all__ = ["Request", "Result", "D4Server", "size_class"]
+__all__ = ["Request", "Result", "D4Server", "size_class", "count_class"]
 
 _ALIGN = 256
 # molecule status bits of the workspace (include/d4amd.h, d4_molecule_status)
@@ -62,6 +77,25 @@ def size_class(n: int) -> int:
     if n <= 256:
         return 256
     return (n + 127) // 128 * 128
+
+
+# a batch of c requests runs padded to the smallest class >= c (at most 1.5 c)
+COUNT_CLASSES = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024)
+
+# the filler molecule of the padding rows (ammonia, bohr): an ordinary small
+# molecule, so a padded batch holds no case the single requests do not
+_FILL_Z = np.array([7, 1, 1, 1], dtype=np.int64)
+_FILL_R = np.array([[0.0, 0.0, 0.0], [1.91, 0.0, -0.68], [-0.96, 1.65, -0.68],
+                    [-0.96, -1.65, -0.68]])
+
+
+def count_class(count: int, limit: int) -> int:
+    """Padded batch size of a batch of ``count`` requests whose bucket admits
+    at most ``limit`` molecules per slot."""
+    for c in COUNT_CLASSES:
+        if c >= count:
+            return min(c, limit)
+    return min(count, limit)
 
 
 def _up(x: int) -> int:
@@ -96,7 +130,8 @@ class Result:
 class _Slot:
     """One in-flight batch position: a stream, a device arena (inputs,
     outputs, workspace) and a pinned host buffer (inputs, then outputs),
-    grown on demand and reused."""
+    grown on demand and reused, and the batch layouts (with their captured
+    graphs) used in it so far."""
 
     def __init__(self, device: torch.device):
         self.device = device
@@ -106,13 +141,93 @@ class _Slot:
         self.event = None
         self.reqs = None
         self.bid = 0
-        self.shape = None
+        self.batch = None
+        self.layouts: dict[tuple, _Batch] = {}
 
     def reserve(self, dev_bytes: int, host_bytes: int) -> None:
+        grown = False
         if self.dev.numel() < dev_bytes:
+            self.dev = None  # (free the old arena first)
             self.dev = torch.empty(dev_bytes, dtype=torch.uint8, device=self.device)
+            grown = True
         if self.host.numel() < host_bytes:
             self.host = torch.empty(host_bytes, dtype=torch.uint8).pin_memory()
+            grown = True
+        if grown:  # views and graphs point into the old buffers
+            self.layouts.clear()
+
+
+class _Batch:
+    """One (count class, padded length, kind) layout in a slot: numpy views
+    of the pinned buffer, tensor views of the device arena, and the HIP graph
+    of the evaluation once captured."""
+
+    def __init__(self, slot: _Slot, nbatch: int, npad: int, forces: bool):
+        lay = _layout(nbatch, npad, forces)
+        self.nbatch, self.npad, self.forces = nbatch, npad, forces
+        self.in_bytes, self.out = lay["in_end"], (lay["energy"], lay["out_end"])
+        self.status = lay["status"]
+        h = slot.host.numpy()
+        bn = nbatch * npad
+
+        def hv(name, nbytes, dtype, shape):
+            return h[lay[name]: lay[name] + nbytes].view(dtype).reshape(shape)
+
+        self.hn = hv("numbers", 8 * bn, np.int64, (nbatch, npad))
+        self.hp = hv("positions", 24 * bn, np.float64, (nbatch, npad, 3))
+        self.hc = hv("charge", 8 * nbatch, np.float64, (nbatch,))
+        self.he = hv("energy", 8 * bn, np.float64, (nbatch, npad))
+        self.hg = hv("grad", 24 * bn, np.float64, (nbatch, npad, 3)) if forces else None
+        self.hs = hv("status", 4 * nbatch, np.int32, (nbatch,))
+        d = slot.dev
+        self.nt = _view(d, lay["numbers"], torch.int64, (nbatch, npad))
+        self.pt = _view(d, lay["positions"], torch.float64, (nbatch, npad, 3))
+        self.ct = _view(d, lay["charge"], torch.float64, (nbatch,))
+        self.ws = d[lay["ws"]: lay["ws"] + lay["ws_bytes"]]
+        self.et = _view(d, lay["energy"], torch.float64, (nbatch, npad))
+        self.gt = _view(d, lay["grad"], torch.float64, (nbatch, npad, 3)) if forces else None
+        self.graph = None
+
+    def pack(self, reqs: list[Request]) -> None:
+        """Requests into rows 0.. of the pinned input buffer (zero padding),
+        the filler molecule into the rest (every time: the layouts of a slot
+        share its buffer)."""
+        hn, hp, hc = self.hn, self.hp, self.hc
+        count = len(reqs)
+        hn.fill(0)
+        hp.fill(0.0)
+        for k, r in enumerate(reqs):
+            n = len(r.numbers)
+            hn[k, :n] = r.numbers
+            hp[k, :n] = r.positions
+            hc[k] = r.charge
+        if count < self.nbatch:
+            hn[count:, :4] = _FILL_Z
+            hp[count:, :4] = _FILL_R
+            hc[count:] = 0.0
+
+    def _call(self, kp) -> None:
+        if self.forces:
+            runtime.energy_gradient(self.nt, self.pt, self.ct, kp, ws=self.ws, energy=self.et,
+                                    grad=self.gt)
+        else:
+            runtime.forward(self.nt, self.pt, self.ct, kp, ws=self.ws, energy=self.et)
+
+    def run(self, kp, stream, graphs: bool) -> None:
+        """The evaluation on ``stream``: graph replay once captured; else
+        eager, then (with ``graphs``) captured for the next batch of this
+        layout."""
+        if self.graph is not None:
+            with torch.cuda.stream(stream):
+                self.graph.replay()
+            return
+        with torch.cuda.stream(stream):
+            self._call(kp)
+        if graphs:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=stream):
+                self._call(kp)
+            self.graph = g
 
 
 @functools.lru_cache(maxsize=8192)
@@ -167,6 +282,8 @@ def _value_error(req: Request) -> str | None:
     """The checks on the request's values: atomic-number range, finiteness."""
     if req.numbers.min() < 0 or req.numbers.max() >= 104:
         return "atomic numbers must lie in 0..103 (0 marks padding)"
+    if not req.numbers.any():
+        return "a request needs at least one atom (all atomic numbers are 0)"
     if not np.isfinite(req.positions).all() or not np.isfinite(req.charge):
         return "positions and charge must be finite"
     return None
@@ -192,7 +309,7 @@ class D4Server:
 
     def __init__(self, param: dict, cutoff: Cutoff | None = None, device="cuda:0",
                  memory_budget: int = 16 << 30, max_inflight: int = 4,
-                 max_batch: int = 1024):
+                 max_batch: int = 1024, merge_kinds: bool = True, graphs: bool = False):
         self.device = torch.device(device)
         self.kp = kernel_params(param, cutoff, None)
         runtime.context(self.device, self.kp.ga, self.kp.gc)  # tables resident now
@@ -200,6 +317,9 @@ class D4Server:
         self.max_inflight = int(max_inflight)
         self.slot_budget = self.budget // self.max_inflight
         self.max_batch = int(max_batch)
+        self.merge_kinds = bool(merge_kinds)
+        self.graphs = bool(graphs)
+        self._limits: dict[tuple, int] = {}
         self.queues: dict[tuple, collections.deque] = collections.defaultdict(collections.deque)
         self.inflight: list[_Slot] = []
         self.results: list[Result] = []
@@ -236,7 +356,7 @@ class D4Server:
         if why is not None:
             self._fail(req, why)
             return
-        self.queues[(req.forces, size_class(len(req.numbers)))].append(req)
+        self.queues[(req.forces or self.merge_kinds, size_class(len(req.numbers)))].append(req)
 
     def pending(self) -> int:
         return sum(len(q) for q in self.queues.values())
@@ -249,76 +369,78 @@ class D4Server:
                 best, key = q[0].arrival, k
         return key
 
+    def _limit(self, npad: int, forces: bool) -> int:
+        """Most molecules of padded length ``npad`` one slot admits."""
+        key = (npad, forces)
+        if key not in self._limits:
+            self._limits[key] = self._admissible(npad, forces, self.max_batch)
+        return self._limits[key]
+
+    def _packed(self, slot: _Slot, reqs: list[Request], npad: int, forces: bool,
+                limit: int) -> _Batch:
+        """The slot's layout for these requests (padded to their count
+        class), with the requests packed into its pinned buffer."""
+        nbatch = count_class(len(reqs), limit)
+        lay = _layout(nbatch, npad, forces)
+        slot.reserve(lay["dev_end"], lay["host_end"])
+        b = slot.layouts.get((nbatch, npad, forces))
+        if b is None:
+            b = slot.layouts[(nbatch, npad, forces)] = _Batch(slot, nbatch, npad, forces)
+        b.pack(reqs)
+        return b
+
+    def _checked(self, slot, reqs: list[Request], npad: int, forces: bool, limit: int):
+        """Pack the requests and check their values on the packed arrays;
+        when that fails (rare), fail the bad requests alone (``_value_error``)
+        and pack the rest once more. Returns (layout, requests), layout None
+        when no request is left."""
+        b = self._packed(slot, reqs, npad, forces, limit)
+        c = len(reqs)
+        hn = b.hn[:c]
+        if (hn.min() >= 0 and hn.max() < 104 and hn.any(axis=1).all()
+                and np.isfinite(b.hp[:c]).all() and np.isfinite(b.hc[:c]).all()):
+            return b, reqs
+        good = []
+        for r in reqs:
+            why = _value_error(r)
+            if why is None:
+                good.append(r)
+            else:
+                self._fail(r, why)
+        if not good:
+            return None, good
+        return self._packed(slot, good, npad, forces, limit), good
+
     def _launch(self, key) -> bool:
         forces, npad = key
         q = self.queues[key]
-        count = self._admissible(npad, forces, min(len(q), self.max_batch))
-        if count == 0:  # submit() rejects such requests; kept as a guard
+        limit = self._limit(npad, forces)
+        if limit == 0:  # submit() rejects such requests; kept as a guard
             raise MemoryError(f"a {npad}-atom request exceeds the per-slot memory budget "
                               f"({self.slot_budget} bytes)")
-        reqs = [q.popleft() for _ in range(count)]
-        zc = np.concatenate([r.numbers for r in reqs])
-        pc = np.concatenate([r.positions for r in reqs])
-        charges = np.fromiter((r.charge for r in reqs), dtype=np.float64, count=count)
-        if (zc.min() < 0 or zc.max() >= 104 or not np.isfinite(pc).all()
-                or not np.isfinite(charges).all()):
-            # rare: find the bad requests, fail them alone, pack the rest
-            good = []
-            for r in reqs:
-                why = _value_error(r)
-                if why is None:
-                    good.append(r)
-                else:
-                    self._fail(r, why)
-            reqs = good
-            count = len(reqs)
-            if count == 0:
-                return False
-            zc = np.concatenate([r.numbers for r in reqs])
-            pc = np.concatenate([r.positions for r in reqs])
-            charges = np.fromiter((r.charge for r in reqs), dtype=np.float64, count=count)
+        reqs = [q.popleft() for _ in range(min(len(q), limit))]
         slot = self._free.pop()
-        lay = _layout(count, npad, forces)
-        slot.reserve(lay["dev_end"], lay["host_end"])
-        # pack into the pinned buffer: zero padding, then scatter every
-        # request's atoms to (row, column) in one vectorised copy
-        lens = np.fromiter((len(r.numbers) for r in reqs), dtype=np.int64, count=count)
-        # flat destination of every atom: row k starts at k * npad
-        start = np.arange(0, count * npad, npad, dtype=np.int64)
-        flat = np.arange(int(lens.sum()), dtype=np.int64) + np.repeat(
-            start - (np.cumsum(lens) - lens), lens)
-        h = slot.host.numpy()
-        hn = h[lay["numbers"]: lay["numbers"] + 8 * count * npad].view(np.int64)
-        hp = h[lay["positions"]: lay["positions"] + 24 * count * npad].view(np.float64)
-        hc = h[lay["charge"]: lay["charge"] + 8 * count].view(np.float64)
-        hn.fill(0)  # padding atoms: Z = 0 at the origin
-        hp.fill(0.0)
-        hn[flat] = zc
-        hp.reshape(-1, 3)[flat] = pc
-        hc[:] = charges
-        with torch.cuda.stream(slot.stream):
-            slot.dev[: lay["in_end"]].copy_(slot.host[: lay["in_end"]], non_blocking=True)
-            nt = _view(slot.dev, lay["numbers"], torch.int64, (count, npad))
-            pt = _view(slot.dev, lay["positions"], torch.float64, (count, npad, 3))
-            ct = _view(slot.dev, lay["charge"], torch.float64, (count,))
-            ws = slot.dev[lay["ws"]: lay["ws"] + lay["ws_bytes"]]
-            energy = _view(slot.dev, lay["energy"], torch.float64, (count, npad))
-            if forces:
-                grad = _view(slot.dev, lay["grad"], torch.float64, (count, npad, 3))
-                runtime.energy_gradient(nt, pt, ct, self.kp, ws=ws, energy=energy, grad=grad)
-            else:
-                runtime.forward(nt, pt, ct, self.kp, ws=ws, energy=energy)
-            lo, hi = lay["energy"], lay["out_end"]
+        b, reqs = self._checked(slot, reqs, npad, forces, limit)
+        if b is None:
+            self._free.append(slot)
+            return False
+        nbatch = b.nbatch
+        st = slot.stream
+        with torch.cuda.stream(st):
+            slot.dev[: b.in_bytes].copy_(slot.host[: b.in_bytes], non_blocking=True)
+        b.run(self.kp, st, self.graphs)
+        with torch.cuda.stream(st):
+            lo, hi = b.out
             slot.host[lo:hi].copy_(slot.dev[lo:hi], non_blocking=True)
             # status words: the first array of the workspace
-            slot.host[lay["status"]: lay["status"] + 4 * count].copy_(
-                ws[: 4 * count], non_blocking=True)
+            slot.host[b.status: b.status + 4 * nbatch].copy_(b.ws[: 4 * nbatch],
+                                                             non_blocking=True)
             slot.event = torch.cuda.Event()
-            slot.event.record(slot.stream)
+            slot.event.record(st)
         self._bid += 1
         slot.bid = self._bid
         slot.reqs = reqs
-        slot.shape = (count, npad, forces, lay)
+        slot.batch = b
         self.inflight.append(slot)
         return True
 
@@ -333,19 +455,13 @@ class D4Server:
                 still.append(slot)
                 continue
             now = time.perf_counter()
-            count, npad, forces, lay = slot.shape
-            h = slot.host.numpy()
-            e = h[lay["energy"]: lay["energy"] + 8 * count * npad].view(np.float64)
-            e = e.reshape(count, npad)
-            g = None
-            if forces:
-                g = h[lay["grad"]: lay["grad"] + 24 * count * npad].view(np.float64)
-                g = g.reshape(count, npad, 3)
-            status = h[lay["status"]: lay["status"] + 4 * count].view(np.int32).copy()
+            b = slot.batch
+            count = len(slot.reqs)
             # one copy of the batch's outputs out of the pinned buffer (reused by
             # the slot's next batch); each result is a view of its rows
-            e = e.copy()
-            g = None if g is None else g.copy()
+            e = b.he[:count].copy()
+            g = None if b.hg is None else b.hg[:count].copy()
+            status = b.hs[:count].copy()
             bad = status & (STATUS_SINGULAR | STATUS_BAD_Z)
             finite = np.isfinite(e).all(axis=1)
             if g is not None:
@@ -356,7 +472,7 @@ class D4Server:
             for k, r in enumerate(slot.reqs):
                 n = len(r.numbers)
                 if ok[k]:
-                    results.append(Result(r.rid, e[k, :n], None if g is None else g[k, :n],
+                    results.append(Result(r.rid, e[k, :n], g[k, :n] if r.forces else None,
                                           r.arrival, now, bid))
                     continue
                 if status[k] & STATUS_SINGULAR:

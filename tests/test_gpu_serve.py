@@ -15,7 +15,7 @@ from oracle import dense
 from tad_dftd4_amd import runtime, synth, tune
 from tad_dftd4_amd.cutoff import Cutoff
 from tad_dftd4_amd.disp import kernel_params
-from tad_dftd4_amd.serve import D4Server, Request, latency_stats
+from tad_dftd4_amd.serve import D4Server, Request, count_class, latency_stats, size_class
 
 pytestmark = pytest.mark.gpu
 
@@ -32,23 +32,46 @@ def _requests(count, seed=7):
     return reqs
 
 
+MODES = {
+    # default: energy-only requests in the energy + forces batches, eager launches
+    "default": {},
+    "graph replay": {"graphs": True},
+    "kinds apart": {"merge_kinds": False},
+}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("shapes", ["builtin", "shipped"])
-def test_stream_alone_vs_mix(shapes):
-    print(f"[case] serving stream, 120 requests, budget 64 MiB, 3 in flight, {shapes} shapes",
-          flush=True)
+def test_stream_alone_vs_mix(shapes, mode):
+    print(f"[case] serving stream, 120 requests, budget 64 MiB, 3 in flight, {shapes} shapes, "
+          f"{mode}", flush=True)
     table = None
     if shapes == "shipped":
         table = tune.ShapeTable.load(Path(tune.__file__).with_name("shapes_gfx950.json"))
     tune.activate(table)
     try:
-        _alone_vs_mix()
+        _alone_vs_mix(MODES[mode])
     finally:
         tune.activate(None)
 
 
-def _alone_vs_mix():
+def _alone(server, req, kp):
+    """The request alone (runtime calls on its own one-molecule batch), in
+    the server's way: energy + forces for every request when it merges the
+    kinds."""
+    numbers, positions, charge = synth.pack(
+        [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda")
+    if req.forces or server.merge_kinds:
+        e, g, _ = runtime.energy_gradient(numbers, positions, charge, kp)
+        return e[0].cpu().numpy(), g[0].cpu().numpy()
+    e, _ = runtime.forward(numbers, positions, charge, kp)
+    return e[0].cpu().numpy(), None
+
+
+def _alone_vs_mix(kwargs):
     cut = Cutoff(disp2=50.0, disp3=25.0, cn_eeq=25.0)
-    server = D4Server(PARAM, cut, memory_budget=64 << 20, max_inflight=3, max_batch=24)
+    server = D4Server(PARAM, cut, memory_budget=64 << 20, max_inflight=3, max_batch=24,
+                      **kwargs)
     reqs = _requests(120)
     arrivals = np.cumsum(np.random.default_rng(3).exponential(2e-4, size=len(reqs)))
     results = server.run_stream(reqs, arrivals)
@@ -59,21 +82,56 @@ def _alone_vs_mix():
     kp = kernel_params(PARAM, cut, None)
     by_id = {r.rid: r for r in results}
     for req in reqs:
-        numbers, positions, charge = synth.pack(
-            [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda")
         res = by_id[req.rid]
+        e, g = _alone(server, req, kp)
+        assert np.array_equal(res.energy, e)
         if req.forces:
-            e, g, _ = runtime.energy_gradient(numbers, positions, charge, kp)
-            assert np.array_equal(res.gradient, g[0].cpu().numpy())
+            assert np.array_equal(res.gradient, g)
         else:
-            e, _ = runtime.forward(numbers, positions, charge, kp)
-        assert np.array_equal(res.energy, e[0].cpu().numpy())
+            assert res.gradient is None
     # and against the oracle for a few of them
     for req in reqs[:6]:
         eo = dense.dftd4(torch.tensor(req.numbers), torch.tensor(req.positions), req.charge,
                          {k: torch.tensor(v, dtype=torch.float64) for k, v in PARAM.items()},
                          cutoff={"disp2": 50.0, "disp3": 25.0, "cn_eeq": 25.0})
         assert np.abs(by_id[req.rid].energy - eo.numpy()).max() < 1e-10
+
+
+def test_graph_replays_match_first_batches():
+    """The same stream served twice by one server: the second pass replays
+    the graphs the first captured (batches of a layout seen before), and
+    every result is bitwise the first pass's, batches padded with the filler
+    molecule included."""
+    print("[case] serving stream twice, graph replay", flush=True)
+    cut = Cutoff(disp2=50.0, disp3=25.0, cn_eeq=25.0)
+    server = D4Server(PARAM, cut, memory_budget=64 << 20, max_inflight=2, max_batch=24,
+                      graphs=True)
+    reqs = _requests(80, seed=17)
+    arrivals = np.cumsum(np.random.default_rng(6).exponential(2e-4, size=len(reqs)))
+    first = server.run_stream(reqs, arrivals)
+    captured = sum(b.graph is not None for s in server._free for b in s.layouts.values())
+    assert captured > 0
+    again = [Request(r.rid, r.numbers, r.positions, r.charge, r.forces) for r in reqs]
+    second = server.run_stream(again, arrivals)
+    replayed = sum(b.graph is not None for s in server._free for b in s.layouts.values())
+    assert replayed >= captured
+    a, b = {r.rid: r for r in first}, {r.rid: r for r in second}
+    for req in reqs:
+        assert a[req.rid].error is None and b[req.rid].error is None
+        assert np.array_equal(a[req.rid].energy, b[req.rid].energy)
+        if req.forces:
+            assert np.array_equal(a[req.rid].gradient, b[req.rid].gradient)
+    # some batches ran with filler rows (a count below its count class)
+    sizes = {r.rid: len(r.numbers) for r in reqs}
+    padded = 0
+    for results in (first, second):
+        batches = {}
+        for r in results:
+            batches.setdefault(r.batch, []).append(size_class(sizes[r.rid]))
+        for npads in batches.values():
+            limit = server._limit(npads[0], True)
+            padded += count_class(len(npads), limit) > len(npads)
+    assert padded > 0
 
 
 def test_two_streams_share_a_context():
@@ -145,12 +203,8 @@ def test_bad_requests_fail_alone(monkeypatch):
     assert len(good) == len(reqs) and all(r.error is None for r in good)
     kp = kernel_params(PARAM, cut, None)
     for req in reqs:
-        numbers, positions, charge = synth.pack(
-            [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda")
+        e, g = _alone(server, req, kp)
+        assert np.array_equal(by_id[req.rid].energy, e)
         if req.forces:
-            e, g, _ = runtime.energy_gradient(numbers, positions, charge, kp)
-            assert np.array_equal(by_id[req.rid].gradient, g[0].cpu().numpy())
-        else:
-            e, _ = runtime.forward(numbers, positions, charge, kp)
-        assert np.array_equal(by_id[req.rid].energy, e[0].cpu().numpy())
+            assert np.array_equal(by_id[req.rid].gradient, g)
     assert latency_stats(server.results)["failed"] == 4

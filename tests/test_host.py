@@ -170,6 +170,85 @@ def test_serving_request_validation():
     assert "memory budget" in request_error(ok, budget - 1)
     assert "integer" in request_error(Request(7, np.array([1.0, 6.0, 8.0]), pos, 0.0, False),
                                       budget)
+    assert "all atomic numbers are 0" in request_error(
+        Request(8, np.zeros(3, dtype=np.int64), pos, 0.0, False), budget)
+
+
+class _HostSlot:
+    """A slot with plain host memory for both buffers (no device)."""
+
+    def __init__(self):
+        import torch
+
+        self.dev = torch.empty(0, dtype=torch.uint8)
+        self.host = torch.empty(0, dtype=torch.uint8)
+        self.layouts = {}
+
+    def reserve(self, dev_bytes, host_bytes):
+        import torch
+
+        if self.dev.numel() < dev_bytes or self.host.numel() < host_bytes:
+            self.dev = torch.empty(max(dev_bytes, self.dev.numel()), dtype=torch.uint8)
+            self.host = torch.empty(max(host_bytes, self.host.numel()), dtype=torch.uint8)
+            self.layouts.clear()
+
+
+def _host_server():
+    from tad_dftd4_amd.serve import D4Server
+
+    server = object.__new__(D4Server)
+    server.results = []
+    return server
+
+
+def test_serving_pack_and_value_checks(monkeypatch):
+    """Packing a batch into its count class: each request in its own row
+    with zero padding, the filler molecule in the rows after the last
+    request; a request with bad values fails alone and the rest are packed
+    again; when the per-request check lets a value through that the batch
+    check rejects (as a planted Z = 110 does, for the device's status word
+    to catch), the batch goes ahead after one repacking."""
+    from tad_dftd4_amd import serve
+    from tad_dftd4_amd.serve import Request
+
+    rng = np.random.default_rng(3)
+    reqs = [Request(k, rng.integers(1, 20, size=n), rng.normal(size=(n, 3)) * 3.0, 0.0, True)
+            for k, n in enumerate((5, 17, 9, 30, 12))]
+    server = _host_server()
+    slot = _HostSlot()
+    b, kept = server._checked(slot, reqs, 32, True, 1024)
+    assert kept == reqs and b.nbatch == 6  # five requests: count class 6
+    for k, r in enumerate(reqs):
+        n = len(r.numbers)
+        assert np.array_equal(b.hn[k, :n], r.numbers) and not b.hn[k, n:].any()
+        assert np.array_equal(b.hp[k, :n], r.positions) and not b.hp[k, n:].any()
+    assert np.array_equal(b.hn[5, :4], [7, 1, 1, 1]) and not b.hn[5, 4:].any()
+    assert b.hc[5] == 0.0
+    bad = Request(9, np.array([6, 120, 1]), np.zeros((3, 3)), 0.0, True)
+    b, kept = server._checked(slot, reqs[:3] + [bad], 32, True, 1024)
+    assert kept == reqs[:3] and b.nbatch == 3
+    assert server.results[-1].rid == 9 and "0..103" in server.results[-1].error
+    planted = Request(10, np.array([6, 110, 1]), np.zeros((3, 3)), 0.0, True)
+    monkeypatch.setattr(serve, "_value_error", lambda r: None)
+    b, kept = server._checked(slot, [planted] + reqs[:1], 32, True, 1024)
+    assert kept == [planted] + reqs[:1] and b.hn[0, 1] == 110
+    b, kept = server._checked(slot, [bad], 32, True, 1024)  # nothing left to fail
+    assert kept == [bad]
+
+
+def test_serving_count_classes():
+    """A batch of c requests is padded to the smallest count class >= c:
+    never more than 1.5 c, never above what a slot admits, and the same
+    class for every c between two classes (one layout and graph each)."""
+    from tad_dftd4_amd.serve import COUNT_CLASSES, count_class
+
+    assert list(COUNT_CLASSES) == sorted(COUNT_CLASSES)
+    for c in range(1, 1025):
+        k = count_class(c, 1024)
+        assert c <= k <= max(c, int(1.5 * c)) and k in COUNT_CLASSES
+        assert count_class(c, 40) == min(k, 40)
+    assert count_class(2000, 4096) == 2000  # beyond the table: no padding
+    assert {count_class(c, 1024) for c in range(97, 129)} == {128}
 
 
 def test_r4r2_gradient_is_refused():
