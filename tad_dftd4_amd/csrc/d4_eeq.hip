@@ -713,11 +713,21 @@ __global__ __launch_bounds__(256) void k_tf_assemble(DevTables t, Params p, Inpu
 }
 
 // Factor the diagonal block at kb and replace it by the inverse of its factor.
-__global__ __launch_bounds__(256) void k_tf_diag(Inputs in, Work w, int kb) {
-  __shared__ double F[kTB * (kTB + 1) / 2];
-  __shared__ double X[kTB][kTB];
-  __shared__ double T[kNB][kTB];
-  __shared__ double s_small[2 * kNB + 2];
+// The tiled kernels take their LDS as dynamic shared memory and at most 64
+// VGPRs (eight waves per SIMD as far as registers go): a work-group then fits
+// beside the three-body kernel's waves (two per SIMD with 220 VGPRs) on any
+// CU instead of waiting until one is nearly empty.
+constexpr size_t kTfDiagLds = sizeof(double) * (kTB * (kTB + 1) / 2 + kTB * kTB + kNB * kTB +
+                                                2 * kNB + 2);
+constexpr size_t kTfTileLds = sizeof(double) * 2 * kTB * (kTB + 1);
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tf_diag(
+    Inputs in, Work w, int kb) {
+  extern __shared__ double s_tf[];
+  double* F = s_tf;                                             // [tri(kTB)]
+  auto X = reinterpret_cast<double (*)[kTB]>(F + kTB * (kTB + 1) / 2);  // [kTB][kTB]
+  auto T = reinterpret_cast<double (*)[kTB]>(&X[kTB][0]);      // [kNB][kTB]
+  double* s_small = &T[kNB][0];                                 // [2 kNB + 2]
   const int64_t m = blockIdx.x;
   if (!tf_active(in, w, m, kb)) return;
   const int tid = threadIdx.x;
@@ -765,9 +775,11 @@ __global__ __launch_bounds__(256) void k_tf_diag(Inputs in, Work w, int kb) {
 }
 
 // Panel tile: rows kb + nb + 64 y ... of the matrix and border become A X^T.
-__global__ __launch_bounds__(256) void k_tf_panel(Inputs in, Work w, int kb) {
-  __shared__ double Xs[kTB][kTB + 1];
-  __shared__ double As[kTB][kTB + 1];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tf_panel(
+    Inputs in, Work w, int kb) {
+  extern __shared__ double s_tf[];
+  auto Xs = reinterpret_cast<double (*)[kTB + 1]>(s_tf);  // [kTB][kTB + 1]
+  auto As = Xs + kTB;                                      // [kTB][kTB + 1]
   const int64_t m = blockIdx.x;
   if (!tf_active(in, w, m, kb)) return;
   const int tid = threadIdx.x;
@@ -813,9 +825,11 @@ __global__ __launch_bounds__(256) void k_tf_panel(Inputs in, Work w, int kb) {
 
 // Trailing tile (I, J): rows r0 + 64 I.. (matrix and border), columns
 // r0 + 64 J.. of the matrix, r0 = kb + 64, lose P_I P_J^T.
-__global__ __launch_bounds__(256) void k_tf_update(Inputs in, Work w, int kb) {
-  __shared__ double Pi[kTB][kTB + 1];
-  __shared__ double Pj[kTB][kTB + 1];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tf_update(
+    Inputs in, Work w, int kb) {
+  extern __shared__ double s_tf[];
+  auto Pi = reinterpret_cast<double (*)[kTB + 1]>(s_tf);  // [kTB][kTB + 1]
+  auto Pj = Pi + kTB;                                      // [kTB][kTB + 1]
   const int64_t m = blockIdx.x;
   if (!tf_active(in, w, m, kb)) return;
   const int tid = threadIdx.x;
@@ -1142,14 +1156,28 @@ hipError_t launch_eeq_tiled(hipStream_t s, const DevTables& t, const Params& p,
   const unsigned B = (unsigned)w.B;
   const unsigned rows = (unsigned)((w.B * (w.N + 2) + kWaves - 1) / kWaves);
   hipLaunchKernelGGL(k_tf_assemble, dim3(rows), dim3(256), 0, s, t, p, in, w);
+  static bool lds_set = false;  // (the attributes are per kernel, set once)
+  if (!lds_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_tf_diag,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kTfDiagLds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_tf_panel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTfTileLds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_tf_update,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTfTileLds);
+    if (e != hipSuccess) return e;
+    lds_set = true;
+  }
   for (int kb = 0; kb < N; kb += kTB) {
-    hipLaunchKernelGGL(k_tf_diag, dim3(B), dim3(256), 0, s, in, w, kb);
+    hipLaunchKernelGGL(k_tf_diag, dim3(B), dim3(256), kTfDiagLds, s, in, w, kb);
     const unsigned ptiles = (unsigned)((N - kb + 1 + kTB - 1) / kTB);
-    hipLaunchKernelGGL(k_tf_panel, dim3(B, ptiles), dim3(256), 0, s, in, w, kb);
+    hipLaunchKernelGGL(k_tf_panel, dim3(B, ptiles), dim3(256), kTfTileLds, s, in, w, kb);
     if (kb + kTB < N) {
       const int nI = (N + 2 - kb - kTB + kTB - 1) / kTB;
-      hipLaunchKernelGGL(k_tf_update, dim3(B, (unsigned)(nI * (nI + 1) / 2)), dim3(256), 0, s,
-                         in, w, kb);
+      hipLaunchKernelGGL(k_tf_update, dim3(B, (unsigned)(nI * (nI + 1) / 2)), dim3(256),
+                         kTfTileLds, s, in, w, kb);
     }
   }
   hipLaunchKernelGGL(k_tf_reduce, dim3(B), dim3(256), 0, s, in, w);
