@@ -128,6 +128,7 @@ struct Work {
   double* pg3;      // [B*P*4] three-body parameter-gradient partials per work-group
   int32_t* eqord;   // [B] molecule order of the per-molecule EEQ kernels (largest first)
   double* bbox2;    // [B] squared diagonal of a molecule's bounding box (N <= kAtm1MaxN)
+  uint64_t* gnear;  // [B*NG8*W8] near bits of 8-atom group pairs (screened kernel)
   int64_t B, N;
 };
 
@@ -170,6 +171,8 @@ __host__ __device__ inline int atm8_splits(int n) {
 // count n alone.
 constexpr int kAtm8sMaxSplits = 64;
 __host__ __device__ inline int64_t n_groups8(int64_t N) { return (N + kAtm8 - 1) / kAtm8; }
+// 64-bit words of one row of the group near bits
+__host__ __device__ inline int64_t near_words8(int64_t N) { return (n_groups8(N) + 63) / 64; }
 __host__ __device__ inline int atm8s_splits(int G, int gx, int smax) {
   const long long r = G - gx, GG = (long long)G * G;
   const long long s = (smax * r * r + GG - 1) / GG;
@@ -317,6 +320,8 @@ inline size_t layout(int64_t B, int64_t N, char* base, Work* w) {
   v.pg3 = (double*)take(32 * size_t(B) * size_t(pgrad_partials(N)));
   v.eqord = (int32_t*)take(4 * size_t(B));
   v.bbox2 = (double*)take(8 * size_t(B));
+  // (every N: the screened kernel also runs for small N when pair tiles are forced)
+  v.gnear = (uint64_t*)take(8 * size_t(B) * size_t(n_groups8(N)) * size_t(near_words8(N)));
   if (w) *w = v;
   return off;
 }

@@ -2116,6 +2116,54 @@ __global__ __launch_bounds__(64) void k_atm8s_index(Work w) {
   }
 }
 
+// Near bits of the 8-atom groups: bit j of row i of molecule m is set when a
+// real atom of group i lies within the three-body cutoff (with a margin for
+// rounding) of a real atom of group j. A triple contributes only through an
+// atom whose two edges are both within the cutoff, so a triple of atoms in
+// groups x, y, z contributes only when two of near(x, y), near(x, z) and
+// near(y, z) are set: the exact group test of the screened kernel (the
+// bounding spheres keep about a quarter more third-atom groups). One wave
+// per (molecule, group i, word of 64 groups j).
+__global__ __launch_bounds__(64) void k_atm8s_near(Params p, Inputs in, Work w) {
+  const int64_t GN = n_groups8(w.N), WN = near_words8(w.N);
+  const int64_t b = blockIdx.x;
+  const int64_t m = b / (GN * WN);
+  const int64_t r = b - m * GN * WN;
+  const int i = (int)(r / WN), word = (int)(r - (int64_t)i * WN);
+  const int lane = threadIdx.x;
+  const int n = w.natom[m];
+  const int64_t* Z = in.numbers + m * w.N;
+  const double* R = in.positions + m * w.N * 3;
+  const double c3 = sqrt(p.cut3sq) * (1.0 + 1e-10) + 1e-10;
+  const double c3sq = c3 * c3;
+  double xi[kAtm8], yi[kAtm8], zi[kAtm8];
+  bool ri[kAtm8];
+#pragma unroll
+  for (int a = 0; a < kAtm8; ++a) {
+    const int at = kAtm8 * i + a;
+    ri[a] = at < n && Z[at] != 0;
+    xi[a] = ri[a] ? R[3 * at] : 0.0;
+    yi[a] = ri[a] ? R[3 * at + 1] : 0.0;
+    zi[a] = ri[a] ? R[3 * at + 2] : 0.0;
+  }
+  const int j = word * 64 + lane;
+  bool near = false;
+  if (j < GN) {
+    for (int c = 0; c < kAtm8; ++c) {
+      const int at = kAtm8 * j + c;
+      if (at >= n || Z[at] == 0) continue;
+      const double x = R[3 * at], y = R[3 * at + 1], z = R[3 * at + 2];
+#pragma unroll
+      for (int a = 0; a < kAtm8; ++a) {
+        const double dx = x - xi[a], dy = y - yi[a], dz = z - zi[a];
+        near |= ri[a] && dx * dx + dy * dy + dz * dz <= c3sq;
+      }
+    }
+  }
+  const unsigned long long bits = __ballot(near);
+  if (lane == 0) w.gnear[(m * GN + i) * WN + word] = bits;
+}
+
 template <bool ENERGY, bool GRAD, bool PG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_atm8s(
     Params p, Inputs in, Work w, int64_t nblocks) {
@@ -2158,6 +2206,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
   const double* S8 = w.gsph8 + 4 * m * GN;
   const double* SX = S8 + 4 * gx;
   const double c3 = sqrt(p.cut3sq) * (1.0 + 1e-10) + 1e-10;  // margin for rounding in the bounds
+  const int64_t WN = near_words8(N);
+  const uint64_t* NEAR = w.gnear + m * GN * WN;
+  const uint64_t* NX = NEAR + gx * WN;
   const int x = gx * T8 + tx;
   Atm8Sums u = {};
   int kept = 0;  // pair tiles of this x-group seen so far
@@ -2169,35 +2220,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
       const int gy = ybase + __builtin_ctzll(ymask);
       ymask &= ymask - 1;
       if (kept++ % S != s) continue;
-      const double* SY = S8 + 4 * gy;
-      const int nxy = sphere_gap(SX, SY) <= c3 ? 1 : 0;
+      const uint64_t* NY = NEAR + gy * WN;
+      const bool nxy = (NX[gy >> 6] >> (gy & 63)) & 1ull;
       const int y = gy * T8 + ty;
       const bool valid = x < y && y < n && Z[x] != 0 && Z[y] != 0;
       const Atm8Pair q = atm8_pair<GRAD, PG>(p, in, w, m, x, y, valid);
       u.ey = u.cy = u.sca = 0.0;
       u.fy0 = u.fy1 = u.fy2 = 0.0;
 
-      // third-atom groups in ascending order, 64 candidates per ballot
-      int zbase = gy;
+      // third-atom groups gz >= gy in ascending order, 64 per word of near
+      // bits: two of near(x, y), near(x, z), near(y, z) set (groups past the
+      // molecule have no bits)
+      int zw = gy >> 6;
       auto zscan = [&]() {
-        const int c = zbase + lane;
-        bool k = false;
-        if (c < G)
-          k = nxy + (sphere_gap(SX, S8 + 4 * c) <= c3 ? 1 : 0) +
-                  (sphere_gap(SY, S8 + 4 * c) <= c3 ? 1 : 0) >=
-              2;
-        return (unsigned long long)__ballot(k);
+        const unsigned long long a = NX[zw], c = NY[zw];
+        unsigned long long k = nxy ? (a | c) : (a & c);
+        if (zw == (gy >> 6)) k &= ~0ull << (gy & 63);
+        return k;
       };
       unsigned long long zmask = zscan();
       auto next_z = [&]() {
         while (true) {
           if (zmask) {
-            const int g = zbase + __builtin_ctzll(zmask);
+            const int g = zw * 64 + __builtin_ctzll(zmask);
             zmask &= zmask - 1;
             return g;
           }
-          zbase += 64;
-          if (zbase >= G) return G;
+          ++zw;
+          if (zw * 64 >= G) return G;
           zmask = zscan();
         }
       };
@@ -2790,6 +2840,8 @@ hipError_t launch_atm8s(hipStream_t s, const Params& p, const Inputs& in, const 
   if (nblocks == 0) return hipSuccess;
   hipError_t e = launch_groups8(s, in, w);
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_atm8s_near, dim3((unsigned)(w.B * n_groups8(w.N) * near_words8(w.N))),
+                     dim3(64), 0, s, p, in, w);
   hipLaunchKernelGGL(k_atm8s_index, dim3((unsigned)((w.B + 63) / 64)), dim3(64), 0, s, w);
   const dim3 g((unsigned)nblocks), blk(64);
   if (energy && grad)
