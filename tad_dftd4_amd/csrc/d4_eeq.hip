@@ -656,6 +656,12 @@ __global__ __launch_bounds__(256) void k_eeq_lu(DevTables t, Params p, Inputs in
 constexpr int kTB = 64;
 constexpr int kEeqTiled = 3;  // adjoint mode: lambda already solved by the tiled kernels
 
+// The tiled solve is a chain of a few hundred short launches that runs beside
+// the screened three-body kernel's waves on the same SIMDs; its waves take
+// issue priority over them, so the chain is not slowed to a third of its
+// speed (the three-body waves use the cycles it leaves).
+__device__ __forceinline__ void tf_priority() { __builtin_amdgcn_s_setprio(2); }
+
 __device__ __forceinline__ double* tf_matrix(const Work& w, int64_t m) {
   return w.lu + (size_t)m * (size_t)(w.N + 1) * (size_t)(w.N + 1);
 }
@@ -723,6 +729,7 @@ constexpr size_t kTfTileLds = sizeof(double) * 2 * kTB * (kTB + 1);
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tf_diag(
     Inputs in, Work w, int kb) {
+  tf_priority();
   extern __shared__ double s_tf[];
   double* F = s_tf;                                             // [tri(kTB)]
   auto X = reinterpret_cast<double (*)[kTB]>(F + kTB * (kTB + 1) / 2);  // [kTB][kTB]
@@ -777,6 +784,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 // Panel tile: rows kb + nb + 64 y ... of the matrix and border become A X^T.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tf_panel(
     Inputs in, Work w, int kb) {
+  tf_priority();
   extern __shared__ double s_tf[];
   auto Xs = reinterpret_cast<double (*)[kTB + 1]>(s_tf);  // [kTB][kTB + 1]
   auto As = Xs + kTB;                                      // [kTB][kTB + 1]
@@ -827,6 +835,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 // r0 + 64 J.. of the matrix, r0 = kb + 64, lose P_I P_J^T.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_tf_update(
     Inputs in, Work w, int kb) {
+  tf_priority();
   extern __shared__ double s_tf[];
   auto Pi = reinterpret_cast<double (*)[kTB + 1]>(s_tf);  // [kTB][kTB + 1]
   auto Pj = Pi + kTB;                                      // [kTB][kTB + 1]
@@ -881,6 +890,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 // right-hand side y_b - mu y_e of the backward substitution (in vec); y_e is
 // kept for the adjoint.
 __global__ __launch_bounds__(256) void k_tf_reduce(Inputs in, Work w) {
+  tf_priority();
   __shared__ double red[2 * kWaves];
   const int64_t m = blockIdx.x;
   const int tid = threadIdx.x;
@@ -919,6 +929,7 @@ __global__ __launch_bounds__(256) void k_tf_reduce(Inputs in, Work w) {
 // work-group y updates v[j] for j in [256 y, 256 y + 256) below kb.
 __global__ __launch_bounds__(256) void k_tf_bsolve(Inputs in, Work w, int kb,
                                                    double* dst_base) {
+  tf_priority();
   __shared__ double xs[kTB];
   const int64_t m = blockIdx.x;
   const int y = (int)blockIdx.y;
@@ -953,6 +964,7 @@ __global__ __launch_bounds__(256) void k_tf_bsolve(Inputs in, Work w, int kb,
 // kb + nb + 256 y .. below the block.
 __global__ __launch_bounds__(256) void k_tf_fsolve(Inputs in, Work w, int kb,
                                                    double* dst_base) {
+  tf_priority();
   __shared__ double ys[kTB];
   const int64_t m = blockIdx.x;
   const int y = (int)blockIdx.y;
@@ -983,6 +995,7 @@ __global__ __launch_bounds__(256) void k_tf_fsolve(Inputs in, Work w, int kb,
 
 // Adjoint set-up for the tiled solve: v = dL/dq on real atoms.
 __global__ __launch_bounds__(256) void k_tf_adj_init(Inputs in, Work w) {
+  tf_priority();
   const int64_t m = blockIdx.x;
   const int64_t N = w.N;
   const int n = w.natom[m];
@@ -995,6 +1008,7 @@ __global__ __launch_bounds__(256) void k_tf_adj_init(Inputs in, Work w) {
 // Between the two adjoint substitutions: w' = L^-1 g is in lam; v = w' - nu y_e
 // with nu = y_e . w' / y_e . y_e.
 __global__ __launch_bounds__(256) void k_tf_adj_mid(Inputs in, Work w) {
+  tf_priority();
   __shared__ double red[2 * kWaves];
   const int64_t m = blockIdx.x;
   const int64_t N = w.N;
