@@ -25,7 +25,8 @@ Other workloads (``--workload``), one JSON line each:
   serving_stream   10,000 requests (n ~ lognormal, median 40, 4..2000 atoms,
                    70 % energy, 30 % energy + forces) served by the
                    continuous-batching loop at 70 % of its measured capacity
-                   with Poisson arrivals: p50 / p99 latency.
+                   with Poisson arrivals: p50 / p99 latency; beside it the
+                   host's per-request latency on the same mix (oracle).
 
 ``--model d4s`` runs the batch workloads with the D4S model (element-pair
 Gaussian weighting factors) instead of D4.
@@ -230,6 +231,46 @@ def cpu_single_64(runs: int = 20) -> dict:
             "what": "1 x 64 atoms, energy + autograd gradient, FP64 dense restatement"}
 
 
+def cpu_serving(reqs, budget_s: float, max_atoms: int = 300) -> dict:
+    """Per-request latency of the serving mix on the host: the oracle (dense
+    FP64 restatement) on the requests in arrival order, energy or energy +
+    autograd forces as each request asks, until ``budget_s`` is spent;
+    requests above ``max_atoms`` are skipped (their n^3 tensors take seconds
+    and gigabytes each)."""
+    import numpy as np
+    import torch
+
+    from oracle import dense
+
+    cpu = host_cpu()
+    par = {k: torch.tensor(v, dtype=torch.float64) for k, v in PARAM.items()}
+    cut = {"disp2": CUT[0], "disp3": CUT[1], "cn_eeq": CUT[2]}
+    lat, skipped = [], 0
+    t0 = time.perf_counter()
+    for r in reqs:
+        if len(r.numbers) > max_atoms:
+            skipped += 1
+            continue
+        nt, pt = torch.tensor(r.numbers), torch.tensor(r.positions)
+        t1 = time.perf_counter()
+        if r.forces:
+            dense.dftd4_with_grad(nt, pt, r.charge, par, cutoff=cut)
+        else:
+            dense.dftd4(nt, pt, r.charge, par, cutoff=cut)
+        lat.append(time.perf_counter() - t1)
+        if time.perf_counter() - t0 > budget_s:
+            break
+    lat = np.array(lat) * 1e3
+    return {"value": round(float(np.median(lat)), 3), "unit": "ms per request (median)",
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "molecules_per_s": round(float(len(lat) / (lat.sum() / 1e3)), 1),
+            "cores": cpu["threads"],
+            "kind": "port", "cpu_model": cpu["model"],
+            "sample": f"the first {len(lat)} requests of the stream in arrival order, one at a "
+                      f"time, as each asks (energy or energy + autograd forces), FP64 dense "
+                      f"restatement; {skipped} requests above {max_atoms} atoms skipped"}
+
+
 def oracle_errors(outs, energy, grad) -> dict:
     """Largest |E - E_oracle| and |g - g_oracle| over the molecules the CPU
     baseline computed, from the timed step's own outputs."""
@@ -285,6 +326,10 @@ def serve_bench(args) -> dict:
     arrivals = np.cumsum(np.random.default_rng(5).exponential(1.0 / rate, size=len(reqs)))
     res = server.run_stream(fresh(), arrivals)
     st = latency_stats(res)
+    extra = {}
+    if args.workload == "serving_stream" and not args.no_baselines:
+        order = np.argsort(arrivals, kind="stable")
+        extra["cpu_baseline"] = cpu_serving([reqs[k] for k in order], args.cpu_seconds)
     return {
         "metric": "p50/p99 serve latency, 10k requests (4-2000 atoms, 70% E / 30% E+F), "
                   "Poisson at 70% of capacity",
@@ -308,6 +353,7 @@ def serve_bench(args) -> dict:
                    "damping": PARAM, "parallelism": "single GPU",
                    "server": {"max_inflight": server.max_inflight,
                               "merge_kinds": server.merge_kinds, "graphs": server.graphs}},
+        **extra,
     }
 
 
