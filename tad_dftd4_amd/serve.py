@@ -5,10 +5,10 @@ Requests (one molecule each: atomic numbers, positions in bohr, total charge,
 energy-only or energy + forces) arrive on the host at any time. The loop
 
 - keeps the element tables resident on the device (one context per device);
-- groups waiting requests by padded size bucket (multiples of 32 atoms up to
-  192, the largest size the single-work-group EEQ solver takes, then 256,
-  then multiples of 128), so a batch holds many requests while padding stays
-  moderate (padding rows exit at once in every kernel); by default
+- groups waiting requests by padded size bucket (64, 128 and 192 atoms, 192
+  being the largest size the single-work-group EEQ solver takes, then 256,
+  then multiples of 128): few queues, so that each comes round often, while
+  padding stays moderate (padding rows exit at once in every kernel); by default
   (``merge_kinds``) energy-only requests share the energy + forces batches of
   their bucket and their forces are dropped, which halves the number of
   queues a request waits behind;
@@ -39,11 +39,11 @@ energy-only or energy + forces) arrive on the host at any time. The loop
   while the rest of its batch succeeds.
 
 The kernels give every molecule a result that does not depend on its batch
-neighbours (nor on the batch size), nor on the padded length within one of
-the buckets up to 192 atoms, so such a request's result is bitwise the same
-alone and in the mix; from 193 atoms on the padded batch takes the spatial
-order and the tiled EEQ and the results agree with the request alone to
-rounding.
+neighbours nor on the batch size, so a request's result is bitwise the same
+alone (a batch of one at its bucket's padded length) and in the mix. Against
+a direct call at the molecule's own length it is bitwise the same where both
+lengths take the same launch shapes (up to 192 atoms with the shipped shape
+table for energy + forces), else equal to rounding.
 """
 
 from __future__ import annotations
@@ -73,7 +73,7 @@ STATUS_BAD_Z = 4
 def size_class(n: int) -> int:
     """Padded length of the batches a molecule of n atoms joins."""
     if n <= 192:
-        return max(32, (n + 31) // 32 * 32)
+        return max(64, (n + 63) // 64 * 64)
     if n <= 256:
         return 256
     return (n + 127) // 128 * 128
@@ -308,7 +308,7 @@ class D4Server:
     """Continuous batching of D4 requests on one device."""
 
     def __init__(self, param: dict, cutoff: Cutoff | None = None, device="cuda:0",
-                 memory_budget: int = 16 << 30, max_inflight: int = 4,
+                 memory_budget: int = 16 << 30, max_inflight: int = 3,
                  max_batch: int = 1024, merge_kinds: bool = True, graphs: bool = False):
         self.device = torch.device(device)
         self.kp = kernel_params(param, cutoff, None)

@@ -1,8 +1,9 @@
 """
 Continuous batching: requests of mixed size and kind, admitted while other
-batches run, give exactly the result each molecule gets alone, with the
-built-in launch shapes and with the shipped tuned shape table (whose buckets
-are the serving loop's padded lengths).
+batches run, give exactly the result each molecule gets alone (a batch of one
+at its bucket's padded length), and the result of a direct call at its own
+length to rounding, with the built-in launch shapes and with the shipped
+tuned shape table.
 """
 
 from pathlib import Path
@@ -55,17 +56,20 @@ def test_stream_alone_vs_mix(shapes, mode):
         tune.activate(None)
 
 
-def _alone(server, req, kp):
-    """The request alone (runtime calls on its own one-molecule batch), in
-    the server's way: energy + forces for every request when it merges the
-    kinds."""
+def _alone(server, req, kp, bucket=True):
+    """The request alone: runtime calls on a one-molecule batch padded to its
+    bucket's length as the server pads it (``bucket=False``: at the
+    molecule's own length), in the server's way (energy + forces for every
+    request when it merges the kinds)."""
+    n = len(req.numbers)
     numbers, positions, charge = synth.pack(
-        [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda")
+        [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda",
+        pad_to=size_class(n) if bucket else None)
     if req.forces or server.merge_kinds:
         e, g, _ = runtime.energy_gradient(numbers, positions, charge, kp)
-        return e[0].cpu().numpy(), g[0].cpu().numpy()
+        return e[0, :n].cpu().numpy(), g[0, :n].cpu().numpy()
     e, _ = runtime.forward(numbers, positions, charge, kp)
-    return e[0].cpu().numpy(), None
+    return e[0, :n].cpu().numpy(), None
 
 
 def _alone_vs_mix(kwargs):
@@ -89,6 +93,10 @@ def _alone_vs_mix(kwargs):
             assert np.array_equal(res.gradient, g)
         else:
             assert res.gradient is None
+        e1, g1 = _alone(server, req, kp, bucket=False)
+        assert np.abs(res.energy - e1).max() < 1e-13
+        if req.forces:
+            assert np.abs(res.gradient - g1).max() < 1e-12
     # and against the oracle for a few of them
     for req in reqs[:6]:
         eo = dense.dftd4(torch.tensor(req.numbers), torch.tensor(req.positions), req.charge,

@@ -33,7 +33,7 @@ from oracle import dense
 from tad_dftd4_amd import runtime, synth, tune
 from tad_dftd4_amd.cutoff import Cutoff
 from tad_dftd4_amd.disp import kernel_params
-from tad_dftd4_amd.serve import D4Server, Request
+from tad_dftd4_amd.serve import D4Server, Request, size_class
 
 pytestmark = pytest.mark.gpu
 
@@ -192,23 +192,26 @@ def test_serving_stream_with_large_requests():
     for req in reqs:
         n = len(req.numbers)
         res = by_id[req.rid]
-        numbers, positions, charge = synth.pack(
-            [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda")
-        # (the server runs energy-only requests in its energy + forces
-        # batches: alone is the same path)
-        e, g, _ = runtime.energy_gradient(numbers, positions, charge, kp)
-        g = g[0].cpu().numpy() if req.forces else None
-        e = e[0].cpu().numpy()
+        # alone: a batch of one at the bucket's padded length (bitwise), and
+        # a direct call at the molecule's own length (to rounding); the
+        # server runs energy-only requests in its energy + forces batches
+        for pad, exact in ((size_class(n), True), (None, False)):
+            numbers, positions, charge = synth.pack(
+                [synth.Molecule(req.numbers, req.positions, req.charge)], device="cuda",
+                pad_to=pad)
+            e, g, _ = runtime.energy_gradient(numbers, positions, charge, kp)
+            g = g[0, :n].cpu().numpy() if req.forces else None
+            e = e[0, :n].cpu().numpy()
+            if exact:
+                assert np.array_equal(res.energy, e), (req.rid, n)
+                if g is not None:
+                    assert np.array_equal(res.gradient, g), (req.rid, n)
+            else:
+                assert np.abs(res.energy - e).max() < 1e-13, (req.rid, n)
+                if g is not None:
+                    assert np.abs(res.gradient - g).max() < 1e-12, (req.rid, n)
         if not req.forces:
             assert res.gradient is None
-        if n <= 192:
-            assert np.array_equal(res.energy, e), req.rid
-            if g is not None:
-                assert np.array_equal(res.gradient, g), req.rid
-        else:
-            assert np.abs(res.energy - e).max() < 1e-13, (req.rid, n)
-            if g is not None:
-                assert np.abs(res.gradient - g).max() < 1e-12, (req.rid, n)
         if n <= 300 and (n > 192 or req.rid % 10 == 0):
             nt, pt = torch.tensor(req.numbers), torch.tensor(req.positions)
             if req.forces:
