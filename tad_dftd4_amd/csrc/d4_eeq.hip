@@ -4,6 +4,10 @@
 // work-group each; larger molecules use the tiled multi-work-group path.
 // Every output element has one writer and every reduction a fixed order.
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "d4_device.h"
 
 namespace d4 {
@@ -1143,17 +1147,29 @@ __global__ __launch_bounds__(1024) void k_eeq_order(Work w) {
 
 }  // namespace
 
+hipError_t ensure_lds(const void* kernel, size_t bytes) {
+  // the limit is a property of the kernel on one device: remember what each
+  // (kernel, device) was given
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, size_t> given;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& have = given[{kernel, dev}];
+  if (bytes <= have) return hipSuccess;
+  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) have = bytes;
+  return e;
+}
+
 // Launch a per-molecule EEQ kernel with `shmem` bytes of dynamic LDS, raising
-// the kernel's LDS limit first when needed (once per kernel and size).
+// the kernel's LDS limit first when needed.
 template <typename... Args>
-hipError_t launch_lds(void (*kernel)(Args...), size_t& configured, unsigned grid,
-                      size_t shmem, hipStream_t s, Args... args) {
-  if (shmem > configured) {
-    hipError_t e = hipFuncSetAttribute((const void*)kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e != hipSuccess) return e;
-    configured = shmem;
-  }
+hipError_t launch_lds(void (*kernel)(Args...), unsigned grid, size_t shmem, hipStream_t s,
+                      Args... args) {
+  hipError_t e = ensure_lds((const void*)kernel, shmem);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), shmem, s, args...);
   return hipGetLastError();
 }
@@ -1170,20 +1186,10 @@ hipError_t launch_eeq_tiled(hipStream_t s, const DevTables& t, const Params& p,
   const unsigned B = (unsigned)w.B;
   const unsigned rows = (unsigned)((w.B * (w.N + 2) + kWaves - 1) / kWaves);
   hipLaunchKernelGGL(k_tf_assemble, dim3(rows), dim3(256), 0, s, t, p, in, w);
-  static bool lds_set = false;  // (the attributes are per kernel, set once)
-  if (!lds_set) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_tf_diag,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)kTfDiagLds);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)k_tf_panel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTfTileLds);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)k_tf_update,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kTfTileLds);
-    if (e != hipSuccess) return e;
-    lds_set = true;
-  }
+  hipError_t e = ensure_lds((const void*)k_tf_diag, kTfDiagLds);
+  if (e == hipSuccess) e = ensure_lds((const void*)k_tf_panel, kTfTileLds);
+  if (e == hipSuccess) e = ensure_lds((const void*)k_tf_update, kTfTileLds);
+  if (e != hipSuccess) return e;
   for (int kb = 0; kb < N; kb += kTB) {
     hipLaunchKernelGGL(k_tf_diag, dim3(B), dim3(256), kTfDiagLds, s, in, w, kb);
     const unsigned ptiles = (unsigned)((N - kb + 1 + kTB - 1) / kTB);
@@ -1206,7 +1212,6 @@ constexpr unsigned kLuGroups = 64;  // work-groups of the LU fallback launch
 
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
                       const Inputs& in, const Work& w, bool /*keep_factor*/) {
-  static size_t configured[2] = {0, 0};
   const unsigned grid = (unsigned)w.B;
   if (grid == 0) return hipSuccess;
   // molecule order of the adjoint (the backward reads the order the forward
@@ -1220,9 +1225,9 @@ hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
     const int mode = eeq_mode(w.N, kEeqVectors);
     const size_t shmem = eeq_shmem(mode, w.N, kEeqVectors);
     if (mode == kEeqAllLds)
-      e = launch_lds(k_eeq_chol<kEeqAllLds>, configured[1], grid, shmem, s, t, p, in, w);
+      e = launch_lds(k_eeq_chol<kEeqAllLds>, grid, shmem, s, t, p, in, w);
     else
-      e = launch_lds(k_eeq_chol<kEeqFactorLds>, configured[0], grid, shmem, s, t, p, in, w);
+      e = launch_lds(k_eeq_chol<kEeqFactorLds>, grid, shmem, s, t, p, in, w);
   }
   if (e != hipSuccess) return e;
   if (tiled)  // (the LDS kernels solve their failed systems themselves)
@@ -1233,7 +1238,6 @@ hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
 
 hipError_t launch_adjoint(hipStream_t s, const DevTables& t, const Params& p,
                           const Inputs& in, const Work& w) {
-  static size_t configured[2] = {0, 0};
   const unsigned grid = (unsigned)w.B;
   if (eeq_tiled(p, w.N)) {
     const int N = (int)w.N;
@@ -1253,8 +1257,8 @@ hipError_t launch_adjoint(hipStream_t s, const DevTables& t, const Params& p,
   const int mode = eeq_mode(w.N, 1);
   const size_t shmem = eeq_shmem(mode, w.N, 1);
   if (mode == kEeqAllLds)
-    return launch_lds(k_adjoint<kEeqAllLds>, configured[1], grid, shmem, s, t, p, in, w);
-  return launch_lds(k_adjoint<kEeqFactorLds>, configured[0], grid, shmem, s, t, p, in, w);
+    return launch_lds(k_adjoint<kEeqAllLds>, grid, shmem, s, t, p, in, w);
+  return launch_lds(k_adjoint<kEeqFactorLds>, grid, shmem, s, t, p, in, w);
 }
 
 }  // namespace d4

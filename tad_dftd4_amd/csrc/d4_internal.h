@@ -338,6 +338,9 @@ struct Inputs {
 
 // Launchers (d4_kernels.hip). All enqueue on `s` and return hipError_t.
 hipError_t launch_prep(hipStream_t s, const Inputs& in, const Work& w);
+// Raise `kernel`'s dynamic-LDS limit to at least `bytes` on the current
+// device (once per kernel, device and size; d4_eeq.hip).
+hipError_t ensure_lds(const void* kernel, size_t bytes);
 hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
                      const Inputs& in, const Work& w);
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,

@@ -318,16 +318,9 @@ hipError_t launch_sort(hipStream_t s, const Inputs& in, const Work& w) {
   int P = 1;
   while (P < w.N) P <<= 1;
   const size_t shmem = sizeof(uint64_t) * (size_t)P;
-  static size_t configured = 0;
-  if (shmem > configured) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_sort,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute((const void*)k_kd_level,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e != hipSuccess) return e;
-    configured = shmem;
-  }
+  hipError_t e = ensure_lds((const void*)k_sort, shmem);
+  if (e == hipSuccess) e = ensure_lds((const void*)k_kd_level, shmem);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_sort, dim3((unsigned)w.B), dim3(kSortThreads), shmem, s, in.numbers,
                      in.positions, w, P);
   // k-d levels while the largest segment (the leftmost; it bounds the
