@@ -193,8 +193,11 @@ int d4_energy_gradient_noalloc(d4_context* ctx, void* stream, int64_t nbatch,
 /* Per-atom properties: D4 coordination numbers, charges, the C6 matrix
  * (nbatch, natoms, natoms) and static polarisabilities (nbatch, natoms).
  * Replaces get_properties() (reference src/tad_dftd4/disp.py:149-197).
- * Any output pointer may be NULL. D4 model only (as the reference, which
- * builds a D4Model there): D4_FLAG_D4S is rejected. */
+ * Any output pointer may be NULL; with only cn_out given, only the
+ * coordination numbers are computed (tad-mctc's cn_d4, which dftd4() takes
+ * as its default CN function, reference src/tad_dftd4/disp.py:30,56-57).
+ * D4 model only (as the reference, which builds a D4Model there):
+ * D4_FLAG_D4S is rejected. */
 int d4_properties(d4_context* ctx, void* stream, int64_t nbatch,
                   int64_t natoms, const int64_t* numbers,
                   const double* positions, const double* charge,

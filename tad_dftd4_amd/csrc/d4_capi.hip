@@ -617,6 +617,18 @@ int d4_properties(d4_context* ctx, void* stream, int64_t nbatch, int64_t natoms,
   if (nbatch == 0 || natoms == 0) return rc;
   Call c = make_call(stream, nbatch, natoms, numbers, positions, charge, nullptr,
                      nullptr, params, workspace);
+  if (q_out == nullptr && c6_out == nullptr && alpha_out == nullptr) {
+    // coordination numbers alone (tad-mctc's cn_d4): no charges or weights
+    if ((rc = run_cn(ctx, c)) != D4_OK) return rc;
+    if (cn_out == nullptr) return D4_OK;
+    if (c.sorted) {
+      D4_TRY(scatter_atoms(c, c.w.cn, cn_out, 1), "scatter coordination numbers");
+    } else {
+      D4_TRY(d4::launch_copy(c.s, c.w.cn, cn_out, nbatch * natoms),
+             "copy coordination numbers");
+    }
+    return D4_OK;
+  }
   if ((rc = run_front(ctx, c)) != D4_OK) return rc;
   if (c.sorted) {
     // C6 is staged in the (unused here) three-body gradient pair array

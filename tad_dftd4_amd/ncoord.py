@@ -29,6 +29,14 @@ def cn_d4(numbers: torch.Tensor, positions: torch.Tensor, **kwargs) -> torch.Ten
     cf = kwargs.pop("counting_function", erf_count)
     if cf is not erf_count or kwargs:
         raise NotImplementedError("cn_d4 supports only the erf counting function")
-    from .disp import get_properties
+    from . import runtime
+    from .damping import RationalDamping
+    from .disp import _prepare, kernel_params
 
-    return get_properties(numbers, positions)[0]
+    model_inst, nums, pos, chg, _, _, lead, n = _prepare(
+        numbers, positions.detach(), 0.0, "d4", None, None, None, None, cn_d4,
+        erf_count, RationalDamping(),
+    )
+    kp = kernel_params({"a1": 0.4, "a2": 5.0}, None, model_inst)
+    cn = runtime.properties(nums, pos, chg, kp, cn_only=True)[0]
+    return cn.reshape(*lead, n).to(positions.dtype)

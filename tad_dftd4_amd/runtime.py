@@ -265,8 +265,9 @@ def energy_gradient_noalloc(numbers, positions, charge, kp: KernelParams, q=None
     return energy, grad
 
 
-def properties(numbers, positions, charge, kp: KernelParams, ws=None):
-    """(cn, q, c6, alpha) as in get_properties."""
+def properties(numbers, positions, charge, kp: KernelParams, ws=None, cn_only: bool = False):
+    """(cn, q, c6, alpha) as in get_properties; with ``cn_only`` just the
+    coordination numbers are computed and (cn, None, None, None) returned."""
     _check_inputs(numbers, positions, charge)
     B, N = numbers.shape
     dev = positions.device
@@ -275,9 +276,9 @@ def properties(numbers, positions, charge, kp: KernelParams, ws=None):
         ws = new_workspace(B, N, dev)
     f64 = dict(dtype=torch.float64, device=dev)
     cn = torch.empty((B, N), **f64)
-    q = torch.empty((B, N), **f64)
-    c6 = torch.empty((B, N, N), **f64)
-    alpha = torch.empty((B, N), **f64)
+    q = None if cn_only else torch.empty((B, N), **f64)
+    c6 = None if cn_only else torch.empty((B, N, N), **f64)
+    alpha = None if cn_only else torch.empty((B, N), **f64)
     p = kp.c(N, grad=False)
     _lib.check(
         _lib.load().d4_properties(

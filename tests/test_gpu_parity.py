@@ -272,6 +272,28 @@ def test_properties_vs_oracle():
     assert torch.allclose(alpha[0].cpu(), alo, rtol=1e-12, atol=0)
 
 
+def test_cn_d4_alone():
+    """cn_d4 runs only the coordination-number kernels: the same bits as the
+    CN of get_properties (which runs the whole chain) for a ragged batch of
+    a 40- and a 300-atom molecule (the second in the spatially sorted
+    layout), and the oracle's CN to 1e-13."""
+    _say("cn_d4 alone")
+    mols = [_mol(81, 40), _mol(82, 300)]
+    numbers, positions, _ = synth.pack(mols, device="cuda")
+    cn = d4.cn_d4(numbers, positions)
+    assert cn.shape == numbers.shape
+    assert torch.equal(cn, d4.get_properties(numbers, positions)[0])
+    for b, mol in enumerate(mols):
+        n = len(mol.numbers)
+        cno = dense.cn_d4(torch.tensor(mol.numbers), torch.tensor(mol.positions))
+        assert torch.allclose(cn[b, :n].cpu(), cno, rtol=1e-13, atol=1e-14)
+        assert torch.all(cn[b, n:] == 0)
+    one = d4.cn_d4(numbers[0, :40], positions[0, :40])
+    assert one.shape == (40,)
+    assert torch.equal(one, d4.get_properties(numbers[0, :40], positions[0, :40])[0])
+    assert torch.allclose(one, cn[0, :40], rtol=1e-13, atol=1e-14)
+
+
 def test_finite_differences_16_atoms():
     """Central differences (h = 1e-5 bohr) of the GPU energy on 16 randomly
     chosen atoms against the GPU gradient."""
