@@ -1260,6 +1260,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
       }
     }
     const double dxy0 = rx0 - ry0, dxy1 = rx1 - ry1, dxy2 = rx2 - ry2;
+    const double asq = a * a;
     double rx4 = 0.0, ry4 = 0.0, Axy = 0.0, Bxy = 0.0;
     if (PG && valid) {
       rx4 = w.rt4[m * N + x];
@@ -1338,18 +1339,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(D4_ATM1_WAV
           }
           const double T = txy * x01.x * y01.x;
           const double X = a + bb - cc, Y = a - bb + cc, W3 = bb + cc - a;
-          const double sabc = X * Y * W3;
+          // energy alone: s = (a^2 - (c - b)^2) W3 (= X Y W3) and
+          // e = P (a b c + 3/8 s) / (a b c (1 + 6 T)) with one Newton step on the
+          // reciprocal (f, g, G2 and K below are then unused)
+          const double sabc =
+              GRAD ? X * Y * W3 : fma(bb - cc, cc - bb, asq) * ((bb + cc) - a);
           const double bc = bb * cc;
           const double abc = a * bc;
           const double opT = fma(6.0, T, 1.0);
-          const double q0 = rcp_nt(abc * opT);
+          const double q0 = GRAD ? rcp_nt(abc * opT) : rcp_newton1(abc * opT);
           const double f = abc * q0;     // 1 / (1 + 6 T)
           const double iabc = opT * q0;  // 1 / (a b c)
           const double G2 = 0.375 * iabc;
           const double G2s = G2 * sabc;
           const double g = 1.0 + G2s;
           const double K = P * f;
-          const double e = K * g;
+          const double e = GRAD ? K * g : (P * q0) * fma(0.375, sabc, abc);
           double sx, sy, sz;
           if (FULL) {
             sx = sy = sz = 2.0;
@@ -1589,7 +1594,7 @@ __device__ __forceinline__ const T* atm8_at(const double* s_zc, uint32_t off) {
 // (no zero distance) with zero pair factors.
 struct Atm8Pair {
   double rx0, rx1, rx2, ry0, ry1, ry2;  // positions
-  double a, ia, a2;                     // r_xy^2, 1 / r_xy^2, 2 r_xy^2
+  double a, ia, a2, asq;                // r_xy^2, 1 / r_xy^2, 2 r_xy^2, r_xy^4
   double txy, pxy, pxy0;                // t_xy, s9 p_xy, p_xy
   double kxy, kyx;                      // d log sqrt|C6(0)_xy| / d CN_x, / d CN_y
   double gx, gy, gxy;                   // dL/dE of x, y and their sum
@@ -1604,7 +1609,7 @@ __device__ __forceinline__ Atm8Pair atm8_pair(const Params& p, const Inputs& in,
   const double* R = in.positions + m * N * 3;
   Atm8Pair q;
   q.rx0 = q.rx1 = q.rx2 = q.ry0 = q.ry1 = q.ry2 = -kAtmFar;
-  q.a = q.ia = 1.0;
+  q.a = q.ia = q.asq = 1.0;
   q.a2 = 2.0;
   q.txy = q.pxy = q.pxy0 = q.kxy = q.kyx = 0.0;
   q.gx = q.gy = q.mxy = 0.0;
@@ -1623,6 +1628,7 @@ __device__ __forceinline__ Atm8Pair atm8_pair(const Params& p, const Inputs& in,
     q.a = d0 * d0 + d1 * d1 + d2 * d2;
     q.ia = 1.0 / q.a;
     q.a2 = 2.0 * q.a;
+    q.asq = q.a * q.a;
     q.txy = pt[(int64_t)x * N + y];
     q.pxy0 = pp[(int64_t)x * N + y];
     q.pxy = p.s9 * q.pxy0;  // s9 folded into the pair factor
@@ -1774,7 +1780,7 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
     // c = r_yz^2); X + Y = 2 a and Y - X = 2 (c - b) give its derivatives
     const double dcb = cc - bb;
     const double X = a - dcb, Y = a + dcb, W3 = (bb + cc) - a;
-    const double C = X * Y;
+    const double C = GRAD ? X * Y : fma(-dcb, dcb, q.asq);  // a^2 - (c - b)^2 = X Y
     const double sabc = C * W3;
     const double bc = bb * cc;
     const double abc = a * bc;
@@ -1786,7 +1792,9 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
     const double G2s = G2 * sabc;
     const double g = 1.0 + G2s;
     const double K = P * f;
-    const double e = K * g;
+    // energy alone: e = P (a b c + 3/8 s) / (a b c (1 + 6 T)), four operations
+    // after the reciprocal instead of seven (f, g, G2 and K are then unused)
+    const double e = GRAD ? K * g : (P * q0) * fma(0.375, sabc, abc);
     double sx, sy, sz;
     if (FULL) {
       sx = sy = sz = 2.0;
