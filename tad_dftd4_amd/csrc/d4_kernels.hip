@@ -1741,6 +1741,8 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
   const uint32_t oz = lrow ^ ((uint32_t)kAtm8RecZ << 5);
   const uint32_t ort = lrow ^ ((uint32_t)kAtm8RecRt << 5);
   const double c3sq = p.cut3sq, alp = p.alp, a1p = p.a1, a2p = p.a2, sq3 = 1.7320508075688772;
+  // alp T f = alp / 6 (1 - f), f = 1 / (1 + 6 T): H below needs no alp T
+  const double alp6 = alp / 6.0, h0 = alp6 - 2.5;
   // four steps per iteration (two with parameter gradients, whose extra
   // sums would not fit the registers), so every ring move below is static
 #ifndef D4_ATM8_UNROLL
@@ -1789,8 +1791,7 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
     const double f = abc * q0;     // 1 / (1 + 6 T)
     const double iabc = opT * q0;  // 1 / (a b c)
     const double G2 = 0.375 * iabc;
-    const double G2s = G2 * sabc;
-    const double g = 1.0 + G2s;
+    const double g = fma(G2, sabc, 1.0);
     const double K = P * f;
     // energy alone: e = P (a b c + 3/8 s) / (a b c (1 + 6 T)), four operations
     // after the reciprocal instead of seven (f, g, G2 and K are then unused)
@@ -1819,11 +1820,16 @@ __device__ __forceinline__ void atm8_chunk(const Params& p, const Atm8Pair& q, A
       // the fused energy + gradient call has dL/dE = 1 for every atom (its
       // triples without atoms have e = 0, whatever the weight)
       constexpr bool kUnit = ENERGY && GRAD && !PG;
-      const double W6 = kUnit ? (FULL ? 6.0 : sx + sy + sz)
-                              : (FULL ? 2.0 * (q.gxy + rz.w)
-                                      : fma(q.gx, sx, fma(q.gy, sy, rz.w * sz)));
+      // with unit weights and every pair within the cutoff the weight is the
+      // constant 6: the caller applies it to the staged gradient sums
+      constexpr bool kSix = kUnit && FULL;
+      const double W6 = kSix ? 1.0
+                             : kUnit ? sx + sy + sz
+                                     : (FULL ? 2.0 * (q.gxy + rz.w)
+                                             : fma(q.gx, sx, fma(q.gy, sy, rz.w * sz)));
       const double We = W6 * e;
-      const double H = fma(g, fma(alp * T, f, -1.5), -G2s);
+      // H = g (alp T f - 3/2) - (g - 1)
+      const double H = fma(g, fma(-alp6, f, h0), 1.0);
       // ds/da = 2 a W3 - X Y, ds/db = X Y + 2 (c - b) W3, ds/dc = X Y - 2 (c - b) W3
       const double dW = dcb * W3;
       const double dsa = fma(q.a2, W3, -C), dsb = fma(2.0, dW, C), dsc = fma(-2.0, dW, C);
@@ -1888,15 +1894,18 @@ __device__ __forceinline__ void atm8_fold_sca(const Atm8Pair& q, Atm8Sums& u) {
 }
 
 // Stage five per-lane values (energy share, gradient at half weight, dL/dCN)
-// in s_red for the sums over rows or columns of the tile.
+// in s_red for the sums over rows or columns of the tile; gs scales the
+// gradient parts (6 where the chunks left out the constant weight 6, else 1).
 template <int NA>
 __device__ __forceinline__ void atm8_stage_sums(double (*s_red)[64], int lane, double e,
-                                                double f0, double f1, double f2, double c) {
+                                                double f0, double f1, double f2, double c,
+                                                double gs = 1.0) {
+  const double g2 = 2.0 * gs;
   s_red[0][lane] = e;
-  s_red[1][lane] = 2.0 * f0;
-  s_red[2][lane] = 2.0 * f1;
-  s_red[3][lane] = 2.0 * f2;
-  s_red[4][lane] = c;
+  s_red[1][lane] = g2 * f0;
+  s_red[2][lane] = g2 * f1;
+  s_red[3][lane] = g2 * f2;
+  s_red[4][lane] = gs * c;
 }
 
 // Parameter-gradient partials of a wave: s9, alp, a1, a2 (the sums carry a
@@ -1949,6 +1958,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
   // within the cutoff: all screening weights are 2 and their masks drop out
   // (the squared box diagonal comes from the coordination-number pass)
   const bool full = w.bbox2[m] <= c3sq;
+  // the weight 6 of the unit-weight chunks of such molecules (see atm8_chunk)
+  const double gs = (ENERGY && GRAD && !PG && full) ? 6.0 : 1.0;
 
   // W-tile lane roles: lane (h, p) = (lane >> 5, lane & 31) owns the pair
   // p = v (v - 1) / 2 + u (u < v) of block 2 t + h; lanes p >= 28 are idle
@@ -2018,7 +2029,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
       if (full) rr.E *= 2.0;
       // lane tx of every row now holds that row's sums for z0 + tx (the
       // gradient parts at half weight); add the 8 rows in order
-      atm8_stage_sums<NA>(s_red, lane, rr.E, rr.F0, rr.F1, rr.F2, rr.D);
+      atm8_stage_sums<NA>(s_red, lane, rr.E, rr.F0, rr.F1, rr.F2, rr.D, gs);
       __syncthreads();
       if (lane < T8 * NA) {
         const int l = lane & (T8 - 1), c = lane >> 3;  // column l holds z0 + l
@@ -2037,8 +2048,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(D4_ATM8_WAVE
     }
     atm8_fold_sca(q, u);
     __syncthreads();
-    atm8_stage_sums<NA>(s_red, lane, u.ex, u.fx0, u.fx1, u.fx2, u.cx);
-    atm8_stage_sums<NA>(s_red2, lane, u.ey, u.fy0, u.fy1, u.fy2, u.cy);
+    atm8_stage_sums<NA>(s_red, lane, u.ex, u.fx0, u.fx1, u.fx2, u.cx, gs);
+    atm8_stage_sums<NA>(s_red2, lane, u.ey, u.fy0, u.fy1, u.fy2, u.cy, gs);
     __syncthreads();
     if (wt) {
       // atom a of block 2 t + hh: x of the pairs (a, v > a), y of (u < a, a)
