@@ -269,7 +269,7 @@ int run_cn(d4_context* ctx, Call& c) {
 int run_front(d4_context* ctx, Call& c) {
   int rc = run_cn(ctx, c);
   if (rc != D4_OK) return rc;
-  D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
+  D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, false), "EEQ charges");
   D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsBoth),
          "reference weights");
   return D4_OK;
@@ -287,7 +287,7 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
     // D4S forms the pair weights inside the pair pass: one fused pass on the
     // call's stream instead of two that would each form them
     if (eeq) {
-      D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
+      D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, false), "EEQ charges");
       D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsBoth),
              "reference weights");
     }
@@ -316,7 +316,10 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
   // (its pivoted-LU fallback runs inside it) and the two-body pass uses
   // one-wave work-groups.
   if (eeq) {
-    D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, true), "EEQ charges");
+    // energies alone: the charge chain (EEQ, weights, two-body) is the shorter
+    // one, so the solve may leave half the CUs to the three-body kernel
+    D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, !grad && !c.p.one_stream),
+           "EEQ charges");
     D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsTwoBody),
            "reference weights");
   }

@@ -337,14 +337,12 @@ __device__ __forceinline__ int strict_row(int e) {
 __device__ void eeq_lu_molecule(const DevTables& t, const Inputs& in, const Work& w, int64_t m,
                                 double* s_val, int* s_idx);
 
-// Two work-groups per CU fit the LDS of a 120-atom factor; keep the register
-// use low enough for two waves per SIMD as well.
+// Charges of molecule m by one work-group (smem: the dynamic LDS of the
+// launch; s_small: two solution blocks, sums and a flag).
 template <int MODE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eeq_chol(
-    DevTables t, Params p, Inputs in, Work w) {
-  extern __shared__ double smem[];
-  __shared__ double s_small[2 * kNB + 2 * kWaves + 2];  // two solution blocks, sums, flag
-  const int64_t m = blockIdx.x;
+__device__ __forceinline__ void eeq_chol_molecule(const DevTables& t, const Params& p,
+                                                  const Inputs& in, const Work& w, int64_t m,
+                                                  double* smem, double* s_small) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t N = w.N;
   const int n = w.natom[m];
@@ -462,6 +460,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   if (MODE != kEeqGlobal) {
     const int nt = tri(n);
     for (int k = tid; k < nt; k += 256) gL[k] = L[k];
+  }
+}
+
+// Two work-groups per CU fit the LDS of a 120-atom factor; keep the register
+// use low enough for two waves per SIMD as well. With `paired` (factors that
+// take a whole CU's LDS), work-group b solves the b-th largest and then the
+// b-th smallest molecule: half the CUs hold the solve, and the others are
+// free from the start for the three-body kernel on the companion stream.
+template <int MODE, bool PAIRED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_eeq_chol(
+    DevTables t, Params p, Inputs in, Work w) {
+  extern __shared__ double smem[];
+  __shared__ double s_small[2 * kNB + 2 * kWaves + 2];
+  const int64_t b = blockIdx.x;
+  if (!PAIRED) {
+    eeq_chol_molecule<MODE>(t, p, in, w, b, smem, s_small);
+    return;
+  }
+  const int64_t b2 = w.B - 1 - b;
+  for (int k = 0; k < (b2 > b ? 2 : 1); ++k) {
+    if (k) __syncthreads();  // the LDS is reused
+    eeq_chol_molecule<MODE>(t, p, in, w, w.eqord[k ? b2 : b], smem, s_small);
   }
 }
 
@@ -1211,7 +1231,7 @@ hipError_t launch_eeq_tiled(hipStream_t s, const DevTables& t, const Params& p,
 constexpr unsigned kLuGroups = 64;  // work-groups of the LU fallback launch
 
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
-                      const Inputs& in, const Work& w, bool /*keep_factor*/) {
+                      const Inputs& in, const Work& w, bool paired) {
   const unsigned grid = (unsigned)w.B;
   if (grid == 0) return hipSuccess;
   // molecule order of the adjoint (the backward reads the order the forward
@@ -1225,9 +1245,11 @@ hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
     const int mode = eeq_mode(w.N, kEeqVectors);
     const size_t shmem = eeq_shmem(mode, w.N, kEeqVectors);
     if (mode == kEeqAllLds)
-      e = launch_lds(k_eeq_chol<kEeqAllLds>, grid, shmem, s, t, p, in, w);
+      e = launch_lds(k_eeq_chol<kEeqAllLds, false>, grid, shmem, s, t, p, in, w);
+    else if (paired)  // one work-group per CU: two molecules each (sizes from k_eeq_order)
+      e = launch_lds(k_eeq_chol<kEeqFactorLds, true>, (grid + 1) / 2, shmem, s, t, p, in, w);
     else
-      e = launch_lds(k_eeq_chol<kEeqFactorLds>, grid, shmem, s, t, p, in, w);
+      e = launch_lds(k_eeq_chol<kEeqFactorLds, false>, grid, shmem, s, t, p, in, w);
   }
   if (e != hipSuccess) return e;
   if (tiled)  // (the LDS kernels solve their failed systems themselves)

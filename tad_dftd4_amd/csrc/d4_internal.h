@@ -343,8 +343,11 @@ hipError_t launch_prep(hipStream_t s, const Inputs& in, const Work& w);
 hipError_t ensure_lds(const void* kernel, size_t bytes);
 hipError_t launch_cn(hipStream_t s, const DevTables& t, const Params& p,
                      const Inputs& in, const Work& w);
+// `paired`: where a molecule's factor fills a CU's LDS, two molecules per
+// work-group (half the CUs), for evaluations whose charge chain is shorter
+// than the three-body chain beside it (energies alone)
 hipError_t launch_eeq(hipStream_t s, const DevTables& t, const Params& p,
-                      const Inputs& in, const Work& w, bool keep_factor);
+                      const Inputs& in, const Work& w, bool paired);
 // Parts of the weights and pair passes: the three-body side (weights and C6
 // at q = 0, independent of the EEQ charges), the two-body side (at the EEQ
 // charges) or both.

@@ -337,3 +337,29 @@ def test_many_elements_vs_oracle():
         assert (g[b, :n].cpu() - go).abs().max().item() < G_TOL, b
         assert (ef[b, :n].cpu() - eo).abs().max().item() < E_TOL, b
         assert (gf[b, :n].cpu() - go).abs().max().item() < G_TOL, b
+
+
+@pytest.mark.parametrize("force_lu", [False, True])
+def test_paired_eeq_at_padded_length_200(force_lu):
+    """At padded length 200 one molecule's EEQ factor fills a CU's LDS; an
+    energy-only evaluation on two streams then solves two molecules per
+    work-group (largest with smallest). Energies are bitwise those of the
+    one-stream evaluation (one molecule per work-group) for an odd batch,
+    through the Cholesky and the pivoted-LU paths, and match the oracle."""
+    _say(f"paired EEQ at N=200, force_lu={force_lu}")
+    sizes = [200, 8, 150, 61, 120, 33, 187]
+    mols = [_mol(90 + k, n) for k, n in enumerate(sizes)]
+    numbers, positions, charge = synth.pack(mols, pad_to=200, device="cuda")
+    kp = kernel_params(PARAM, _cut(), None)
+    kp.force_lu = force_lu
+    e2, ws2 = runtime.forward(numbers, positions, charge, kp)
+    kp.one_stream = True
+    e1, ws1 = runtime.forward(numbers, positions, charge, kp)
+    st2 = runtime.molecule_status(ws2, len(mols))
+    assert np.array_equal(st2, runtime.molecule_status(ws1, len(mols)))
+    if force_lu:
+        assert np.all(st2 == 1)
+    assert torch.equal(e1, e2)
+    for k in (1, 4):
+        eo, _ = _oracle(mols[k], grad=False)
+        assert (e2[k, :sizes[k]].cpu() - eo).abs().max().item() < E_TOL
