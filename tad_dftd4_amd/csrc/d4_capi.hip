@@ -283,9 +283,15 @@ int run_front(d4_context* ctx, Call& c) {
 // EEQ adjoint (which needs only dL/dq of the two-body term) also runs before
 // the join.
 int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double* g_out) {
-  if (c.p.d4s) {
-    // D4S forms the pair weights inside the pair pass: one fused pass on the
-    // call's stream instead of two that would each form them
+  // D4S forms the pair weights inside the pair pass. Up to 160 atoms one fused
+  // pass on the call's stream, which forms them once, is faster (1024 x 20-120
+  // atoms: 1.45 against 1.64 ms with gradients); for longer molecules the
+  // three-body term dominates and the split below, which forms the q = 0 and
+  // the q weights in separate passes but runs the three-body side beside the
+  // charges, wins (256 x 8-200 atoms: 1.18 against 1.34 ms). Both give the
+  // same bits.
+  constexpr int64_t kD4sFusedMaxN = 160;
+  if (c.p.d4s && c.w.N <= kD4sFusedMaxN) {
     if (eeq) {
       D4_RUN(D4_SLOT_EEQ, d4::launch_eeq(c.s, ctx->t, c.p, c.in, c.w, false), "EEQ charges");
       D4_RUN(D4_SLOT_WEIGHTS, d4::launch_weights(c.s, ctx->t, c.p, c.w, c.in, d4::kPairsBoth),
