@@ -49,7 +49,8 @@ GA, GC, WF = 3.0, 2.0, 6.0
 
 
 def _t(x, dtype=torch.float64) -> Tensor:
-    return torch.as_tensor(np.asarray(x), dtype=dtype)
+    # a copy: the table arrays are read-only views of the loaded file
+    return torch.as_tensor(np.array(x), dtype=dtype)
 
 
 class Tables:
