@@ -24,11 +24,14 @@ def erf_count(r: torch.Tensor, r0: torch.Tensor, kcn: float = defaults.D4_KCN) -
 
 def cn_d4(numbers: torch.Tensor, positions: torch.Tensor, **kwargs) -> torch.Tensor:
     """D4 coordination numbers of a (batch of) molecule(s) (HIP kernels).
-    Keyword arguments other than ``counting_function=erf_count`` are not
-    supported."""
+    Keyword arguments: ``counting_function`` (only ``erf_count``) and
+    ``cutoff`` (bohr; default 30, pairs beyond it do not count); the other
+    keywords of tad-mctc's ``cn_d4`` (radii, electronegativities, constants)
+    are not supported."""
     cf = kwargs.pop("counting_function", erf_count)
+    cutoff = kwargs.pop("cutoff", None)
     if cf is not erf_count or kwargs:
-        raise NotImplementedError("cn_d4 supports only the erf counting function")
+        raise NotImplementedError("cn_d4 supports only the erf counting function and a cutoff")
     from . import runtime
     from .damping import RationalDamping
     from .disp import _prepare, kernel_params
@@ -38,5 +41,7 @@ def cn_d4(numbers: torch.Tensor, positions: torch.Tensor, **kwargs) -> torch.Ten
         erf_count, RationalDamping(),
     )
     kp = kernel_params({"a1": 0.4, "a2": 5.0}, None, model_inst)
+    if cutoff is not None:
+        kp.cutoff_cn = float(cutoff)
     cn = runtime.properties(nums, pos, chg, kp, cn_only=True)[0]
     return cn.reshape(*lead, n).to(positions.dtype)

@@ -288,6 +288,12 @@ def test_cn_d4_alone():
         cno = dense.cn_d4(torch.tensor(mol.numbers), torch.tensor(mol.positions))
         assert torch.allclose(cn[b, :n].cpu(), cno, rtol=1e-13, atol=1e-14)
         assert torch.all(cn[b, n:] == 0)
+    short = d4.cn_d4(numbers, positions, cutoff=9.0)
+    for b, mol in enumerate(mols):
+        n = len(mol.numbers)
+        cno = dense.cn_d4(torch.tensor(mol.numbers), torch.tensor(mol.positions), cutoff=9.0)
+        assert torch.allclose(short[b, :n].cpu(), cno, rtol=1e-13, atol=1e-14)
+    assert (short - cn).abs().max().item() > 1e-6  # the cutoff removes pairs
     one = d4.cn_d4(numbers[0, :40], positions[0, :40])
     assert one.shape == (40,)
     assert torch.equal(one, d4.get_properties(numbers[0, :40], positions[0, :40])[0])
