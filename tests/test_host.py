@@ -115,6 +115,15 @@ def test_boundary_errors_before_device_work():
     # CPU tensors are rejected: there is no CPU fallback
     with pytest.raises(RuntimeError):
         d4.dftd4(numbers, positions, 0.0, par)
+    # cn_d4: only erf counting and a cutoff; shapes checked; no CPU fallback
+    with pytest.raises(NotImplementedError):
+        d4.cn_d4(numbers, positions, counting_function=lambda r, r0: r)
+    with pytest.raises(NotImplementedError):
+        d4.cn_d4(numbers, positions, kcn=7.0)
+    with pytest.raises(ValueError):
+        d4.cn_d4(numbers, torch.zeros(3, 3))
+    with pytest.raises(RuntimeError):
+        d4.cn_d4(numbers, positions, cutoff=10.0)
 
 
 def test_cutoff_and_model_objects():
