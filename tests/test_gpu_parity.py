@@ -369,3 +369,10 @@ def test_paired_eeq_at_padded_length_200(force_lu):
     for k in (1, 4):
         eo, _ = _oracle(mols[k], grad=False)
         assert (e2[k, :sizes[k]].cpu() - eo).abs().max().item() < E_TOL
+    # charges given by the caller: both launches copy them per molecule
+    qg = torch.where(numbers != 0, 0.01 * (numbers % 5 - 2).to(F64), 0.0)
+    kp.one_stream = False
+    eq2, _ = runtime.forward(numbers, positions, charge, kp, q=qg)
+    kp.one_stream = True
+    eq1, _ = runtime.forward(numbers, positions, charge, kp, q=qg)
+    assert torch.equal(eq1, eq2) and not torch.equal(eq1, e1)
