@@ -299,6 +299,15 @@ int run_split(d4_context* ctx, Call& c, bool eeq, bool energy, bool grad, double
     }
     D4_RUN(D4_SLOT_PAIRS, d4::launch_pairs(c.s, ctx->t, c.p, c.in, c.w, grad, g_out, d4::kPairsBoth),
            "pair pass");
+    // the three-body term and the EEQ adjoint both need only the pair pass
+    if (grad && !c.p.one_stream) {
+      D4_TRY(fork_aux(ctx, c), "fork companion stream");
+      D4_RUN_AUX(D4_SLOT_ATM, d4::launch_atm(c.aux.s, c.p, c.in, c.w, energy, grad),
+                 "three-body term");
+      D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
+      D4_TRY(join_aux(c), "join companion stream");
+      return D4_OK;
+    }
     if (grad)
       D4_RUN(D4_SLOT_ADJOINT, d4::launch_adjoint(c.s, ctx->t, c.p, c.in, c.w), "EEQ adjoint");
     D4_RUN(D4_SLOT_ATM, d4::launch_atm(c.s, c.p, c.in, c.w, energy, grad), "three-body term");
