@@ -475,7 +475,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   __shared__ double s_small[2 * kNB + 2 * kWaves + 2];
   const int64_t b = blockIdx.x;
   if (!PAIRED) {
-    eeq_chol_molecule<MODE>(t, p, in, w, b, smem, s_small);
+    // smallest molecules first: their solves end early and leave their CUs
+    // to the three-body side on the companion stream (the batch order and
+    // largest first were both slower beside it)
+    eeq_chol_molecule<MODE>(t, p, in, w, w.eqord[w.B - 1 - b], smem, s_small);
     return;
   }
   const int64_t b2 = w.B - 1 - b;
